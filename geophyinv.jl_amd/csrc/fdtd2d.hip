@@ -1,0 +1,322 @@
+// 2-D staggered-grid velocity-stress kernels for gfx950.
+//
+// One launch per half step.  The velocity pass fuses what the reference runs as
+// compute_dp!/compute_dstress!, the CPML memory strips, compute_v! and the rigid-face
+// kernels (src/fdtd/advance_acou.jl:11-58, src/fdtd/advance_elastic.jl:534-608,
+// src/fdtd/cpml.jl:158-215, src/fdtd/dirichlet.jl:3-78); the stress pass fuses
+// compute_dv!, the CPML strips, compute_p!/compute_stressii!/compute_stressij! and the
+// free surface (src/fdtd/advance_acou.jl:59-96, src/fdtd/advance_elastic.jl:609-682).
+// Derivatives never leave registers.  Arithmetic keeps the reference's FP32 evaluation
+// order (the library is built with -ffp-contract=off), so results match the unfused
+// CPU restatement bit for bit.
+//
+// Layout: every wavefield and coefficient array of a model uses the same column pitch
+// `ld` (floats), z contiguous.  A thread owns one derivative-grid point (i, j); lanes of
+// a wavefront run along z so every load and store is a coalesced 256-byte row segment.
+#include "gpi_internal.h"
+#include "stencil.h"
+
+namespace gpi {
+
+namespace {
+
+struct VelParams2D {
+    const float* s0;   // acoustic p | elastic tauxx
+    const float* s1;   // elastic tauzz
+    const float* s2;   // elastic tauxz
+    float* vx;
+    float* vz;
+    const float* cx;   // dtinvavxirho
+    const float* cz;   // dtinvavzirho
+    Pml qx0, qx1;      // x-derivative CPML: acoustic dpdx | elastic dtauxxdx, dtauxzdx
+    Pml qz0, qz1;      // z-derivative CPML: acoustic dpdz | elastic dtauzzdz, dtauxzdz
+    int nz, nx, ld, npml;
+    uint32_t pml_mask, rigid_mask;
+    float dxI, dzI;
+};
+
+struct StrParams2D {
+    const float* vx;
+    const float* vz;
+    float* s0;         // acoustic p | elastic tauxx
+    float* s1;         // elastic tauzz
+    float* s2;         // elastic tauxz
+    const float* c0;   // acoustic dtK | elastic dtM
+    const float* c1;   // elastic dtlambda
+    const float* c2;   // elastic dtavmu
+    Pml qx0, qx1;      // acoustic dvxdx | elastic dvxdx, dvzdx
+    Pml qz0, qz1;      // acoustic dvzdz | elastic dvzdz, dvxdz
+    int nz, nx, ld, npml;
+    uint32_t pml_mask, free_mask;
+    float dxI, dzI;
+};
+
+// CPML memory update of cpml.jl:178-180 on a derivative that varies along x (column j of
+// a derivative grid with q.nd columns).  Min face first, then max face, as the reference
+// launches memorynpx! for xmin before xmax.
+__device__ __forceinline__ float cpml_x(float d, const Pml& q, int i, int j, int npml,
+                                        uint32_t mask)
+{
+    if ((mask & GPI_FACE(2, 0)) && j < npml) {
+        float* m = q.psi + i + q.ps2 * j;
+        float mv = q.b[j] * *m + q.a[j] * d;
+        *m = mv;
+        d = d * q.kI[j] + mv;
+    }
+    if ((mask & GPI_FACE(2, 1)) && j >= q.nd - npml) {
+        int c = j - (q.nd - npml) + npml;
+        float* m = q.psi + i + q.ps2 * c;
+        float mv = q.b[c] * *m + q.a[c] * d;
+        *m = mv;
+        d = d * q.kI[c] + mv;
+    }
+    return d;
+}
+
+// Same along z (row i of a derivative grid with q.nd rows); psi rows are the 2*npml strip.
+__device__ __forceinline__ float cpml_z(float d, const Pml& q, int i, int j, int npml,
+                                        uint32_t mask)
+{
+    if ((mask & GPI_FACE(0, 0)) && i < npml) {
+        float* m = q.psi + i + q.ps2 * j;
+        float mv = q.b[i] * *m + q.a[i] * d;
+        *m = mv;
+        d = d * q.kI[i] + mv;
+    }
+    if ((mask & GPI_FACE(0, 1)) && i >= q.nd - npml) {
+        int r = i - (q.nd - npml) + npml;
+        float* m = q.psi + r + q.ps2 * j;
+        float mv = q.b[r] * *m + q.a[r] * d;
+        *m = mv;
+        d = d * q.kI[r] + mv;
+    }
+    return d;
+}
+
+template <int O>
+__device__ __forceinline__ float dz_at(const float* a, int off)
+{
+    float v[O];
+#pragma unroll
+    for (int k = 0; k < O; ++k) v[k] = a[off + k];
+    return stencil<O>(v);
+}
+
+template <int O>
+__device__ __forceinline__ float dx_at(const float* a, int off, int ld)
+{
+    float v[O];
+#pragma unroll
+    for (int k = 0; k < O; ++k) v[k] = a[off + k * ld];
+    return stencil<O>(v);
+}
+
+// ---------------------------------------------------------------------------------------
+// acoustic
+// ---------------------------------------------------------------------------------------
+
+template <int O>
+__global__ __launch_bounds__(256) void vel2d_acoustic(VelParams2D P)
+{
+    constexpr int H = O - 1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int ld = P.ld;
+    // vx[inn] += dtinvavxirho * dpdx, dpdx = D_xi(p) * dxI on (nz-2h) x (nx-h)
+    if (i < P.nz - 2 * H && j < P.nx - H) {
+        float d = dx_at<O>(P.s0, (i + H) + ld * j, ld) * P.dxI;
+        d = cpml_x(d, P.qx0, i, j, P.npml, P.pml_mask);
+        const int o = (i + H) + ld * (j + H);
+        const float nv = P.vx[o] + P.cx[i + ld * j] * d;
+        P.vx[o] = nv;
+        // rigid x faces: the only nonzero ghost is the mirror of the first/last interior
+        // column (dirichlet.jl ighostmin/ighostmax with ifd = 1)
+        if ((P.rigid_mask & GPI_FACE(2, 0)) && j == 0) P.vx[(i + H)] = -nv;
+        if ((P.rigid_mask & GPI_FACE(2, 1)) && j == P.nx - H - 1)
+            P.vx[(i + H) + ld * (P.nx + H - 1)] = -nv;
+    }
+    // vz[inn] += dtinvavzirho * dpdz, dpdz = D_zi(p) * dzI on (nz-h) x (nx-2h)
+    if (i < P.nz - H && j < P.nx - 2 * H) {
+        float d = dz_at<O>(P.s0, i + ld * (j + H)) * P.dzI;
+        d = cpml_z(d, P.qz0, i, j, P.npml, P.pml_mask);
+        const int o = (i + H) + ld * (j + H);
+        const float nv = P.vz[o] + P.cz[i + ld * j] * d;
+        P.vz[o] = nv;
+        if ((P.rigid_mask & GPI_FACE(0, 0)) && i == 0) P.vz[ld * (j + H)] = -nv;
+        if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1)
+            P.vz[(P.nz + H - 1) + ld * (j + H)] = -nv;
+    }
+}
+
+template <int O>
+__global__ __launch_bounds__(256) void str2d_acoustic(StrParams2D P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= P.nz || j >= P.nx) return;
+    const int ld = P.ld;
+    const int o = i + ld * j;
+    float dvx = dx_at<O>(P.vx, o, ld) * P.dxI;       // D_xa(vx)
+    dvx = cpml_x(dvx, P.qx0, i, j, P.npml, P.pml_mask);
+    float dvz = dz_at<O>(P.vz, o) * P.dzI;           // D_za(vz)
+    dvz = cpml_z(dvz, P.qz0, i, j, P.npml, P.pml_mask);
+    P.s0[o] = P.s0[o] + (dvx + dvz) * P.c0[o];
+}
+
+// ---------------------------------------------------------------------------------------
+// elastic (P-SV)
+// ---------------------------------------------------------------------------------------
+
+template <int O>
+__global__ __launch_bounds__(256) void vel2d_elastic(VelParams2D P)
+{
+    constexpr int H = O - 1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int ld = P.ld;
+    // vx[inn] -= dtinvavxirho*(dtauxxdx + dtauxzdz) on (nz-2h) x (nx-h)
+    if (i < P.nz - 2 * H && j < P.nx - H) {
+        float d1 = dx_at<O>(P.s0, (i + H) + ld * j, ld) * P.dxI;   // D_xi(tauxx)
+        d1 = cpml_x(d1, P.qx0, i, j, P.npml, P.pml_mask);
+        float d2 = dz_at<O>(P.s2, i + ld * j) * P.dzI;              // D_za(tauxz)
+        d2 = cpml_z(d2, P.qz1, i, j, P.npml, P.pml_mask);
+        const int o = (i + H) + ld * (j + H);
+        const float nv = P.vx[o] - P.cx[i + ld * j] * (d1 + d2);
+        P.vx[o] = nv;
+        if ((P.rigid_mask & GPI_FACE(2, 0)) && j == 0) P.vx[(i + H)] = -nv;
+        if ((P.rigid_mask & GPI_FACE(2, 1)) && j == P.nx - H - 1)
+            P.vx[(i + H) + ld * (P.nx + H - 1)] = -nv;
+    }
+    // vz[inn] -= dtinvavzirho*(dtauxzdx + dtauzzdz) on (nz-h) x (nx-2h)
+    if (i < P.nz - H && j < P.nx - 2 * H) {
+        float d1 = dx_at<O>(P.s2, i + ld * j, ld) * P.dxI;          // D_xa(tauxz)
+        d1 = cpml_x(d1, P.qx1, i, j, P.npml, P.pml_mask);
+        float d2 = dz_at<O>(P.s1, i + ld * (j + H)) * P.dzI;        // D_zi(tauzz)
+        d2 = cpml_z(d2, P.qz0, i, j, P.npml, P.pml_mask);
+        const int o = (i + H) + ld * (j + H);
+        const float nv = P.vz[o] - P.cz[i + ld * j] * (d1 + d2);
+        P.vz[o] = nv;
+        if ((P.rigid_mask & GPI_FACE(0, 0)) && i == 0) P.vz[ld * (j + H)] = -nv;
+        if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1)
+            P.vz[(P.nz + H - 1) + ld * (j + H)] = -nv;
+    }
+}
+
+template <int O>
+__global__ __launch_bounds__(256) void str2d_elastic(StrParams2D P)
+{
+    constexpr int H = O - 1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int ld = P.ld;
+    const bool fs = (P.free_mask & GPI_FACE(0, 0)) != 0;
+    if (i < P.nz && j < P.nx) {
+        const int o = i + ld * j;
+        float dvx = dx_at<O>(P.vx, o, ld) * P.dxI;   // D_xa(vx)
+        dvx = cpml_x(dvx, P.qx0, i, j, P.npml, P.pml_mask);
+        float dvz = dz_at<O>(P.vz, o) * P.dzI;       // D_za(vz)
+        dvz = cpml_z(dvz, P.qz0, i, j, P.npml, P.pml_mask);
+        const float m = P.c0[o], l = P.c1[o];
+        P.s0[o] = (P.s0[o] - (m * dvx)) - (l * dvz);
+        const float tzz = (P.s1[o] - (m * dvz)) - (l * dvx);
+        // free surface at zmin: tauzz[1,:] = -tauzz[2,:] (advance_elastic.jl:227-230)
+        if (!(fs && i == 0)) P.s1[o] = tzz;
+        if (fs && i == 1) P.s1[ld * j] = -tzz;
+    }
+    // tauxz -= dtavmu*(dvxdz + dvzdx) on (nz-h) x (nx-h)
+    if (i < P.nz - H && j < P.nx - H) {
+        const int o = i + ld * j;
+        float a = dz_at<O>(P.vx, i + ld * (j + H)) * P.dzI;   // D_zi(vx)
+        a = cpml_z(a, P.qz1, i, j, P.npml, P.pml_mask);
+        float b = dx_at<O>(P.vz, (i + H) + ld * j, ld) * P.dxI;  // D_xi(vz)
+        b = cpml_x(b, P.qx1, i, j, P.npml, P.pml_mask);
+        const float t = P.s2[o] - P.c2[o] * (a + b);
+        P.s2[o] = (fs && i == 0) ? 0.0f : t;   // tauxz[1,:] = 0
+    }
+}
+
+template <int O>
+void launch_vel(const Model& m, hipStream_t s, const VelParams2D& P)
+{
+    constexpr int H = O - 1;
+    dim3 blk(64, 4);
+    dim3 grd((P.nz - H + 63) / 64, (P.nx - H + 3) / 4);
+    if (m.g.physics == 0) hipLaunchKernelGGL(vel2d_acoustic<O>, grd, blk, 0, s, P);
+    else hipLaunchKernelGGL(vel2d_elastic<O>, grd, blk, 0, s, P);
+}
+
+template <int O>
+void launch_str(const Model& m, hipStream_t s, const StrParams2D& P)
+{
+    dim3 blk(64, 4);
+    dim3 grd((P.nz + 63) / 64, (P.nx + 3) / 4);
+    if (m.g.physics == 0) hipLaunchKernelGGL(str2d_acoustic<O>, grd, blk, 0, s, P);
+    else hipLaunchKernelGGL(str2d_elastic<O>, grd, blk, 0, s, P);
+}
+
+}  // namespace
+
+void velocity_pass_2d(const Model& m, hipStream_t s)
+{
+    VelParams2D P{};
+    const bool ac = m.g.physics == 0;
+    P.s0 = m.field[ac ? GPI_P : GPI_TAUXX];
+    P.s1 = ac ? nullptr : m.field[GPI_TAUZZ];
+    P.s2 = ac ? nullptr : m.field[GPI_TAUXZ];
+    P.vx = m.field[GPI_VX];
+    P.vz = m.field[GPI_VZ];
+    P.cx = m.coef[GPI_C_DTINVAVXIRHO];
+    P.cz = m.coef[GPI_C_DTINVAVZIRHO];
+    P.qx0 = m.pml[ac ? GPI_DPDX : GPI_DTAUXXDX];
+    P.qx1 = ac ? Pml{} : m.pml[GPI_DTAUXZDX];
+    P.qz0 = m.pml[ac ? GPI_DPDZ : GPI_DTAUZZDZ];
+    P.qz1 = ac ? Pml{} : m.pml[GPI_DTAUXZDZ];
+    P.nz = (int)m.g.n[0];
+    P.nx = (int)m.g.n[2];
+    P.ld = (int)m.pitch0;
+    P.npml = m.g.npml;
+    P.pml_mask = m.g.pml_mask;
+    P.rigid_mask = m.g.rigid_mask;
+    P.dxI = m.g.dsI[2];
+    P.dzI = m.g.dsI[0];
+    switch (m.g.order) {
+    case 2: launch_vel<2>(m, s, P); break;
+    case 4: launch_vel<4>(m, s, P); break;
+    case 6: launch_vel<6>(m, s, P); break;
+    default: launch_vel<8>(m, s, P); break;
+    }
+}
+
+void stress_pass_2d(const Model& m, hipStream_t s)
+{
+    StrParams2D P{};
+    const bool ac = m.g.physics == 0;
+    P.vx = m.field[GPI_VX];
+    P.vz = m.field[GPI_VZ];
+    P.s0 = m.field[ac ? GPI_P : GPI_TAUXX];
+    P.s1 = ac ? nullptr : m.field[GPI_TAUZZ];
+    P.s2 = ac ? nullptr : m.field[GPI_TAUXZ];
+    P.c0 = m.coef[ac ? GPI_C_DTK : GPI_C_DTM];
+    P.c1 = ac ? nullptr : m.coef[GPI_C_DTLAMBDA];
+    P.c2 = ac ? nullptr : m.coef[GPI_C_DTAVMU];
+    P.qx0 = m.pml[GPI_DVXDX];
+    P.qx1 = ac ? Pml{} : m.pml[GPI_DVZDX];
+    P.qz0 = m.pml[GPI_DVZDZ];
+    P.qz1 = ac ? Pml{} : m.pml[GPI_DVXDZ];
+    P.nz = (int)m.g.n[0];
+    P.nx = (int)m.g.n[2];
+    P.ld = (int)m.pitch0;
+    P.npml = m.g.npml;
+    P.pml_mask = m.g.pml_mask;
+    P.free_mask = m.g.free_mask;
+    P.dxI = m.g.dsI[2];
+    P.dzI = m.g.dsI[0];
+    switch (m.g.order) {
+    case 2: launch_str<2>(m, s, P); break;
+    case 4: launch_str<4>(m, s, P); break;
+    case 6: launch_str<6>(m, s, P); break;
+    default: launch_str<8>(m, s, P); break;
+    }
+}
+
+}  // namespace gpi

@@ -1,0 +1,152 @@
+// Source injection and receiver sampling kernels.
+//
+// Injection replaces the reference's two full-grid passes per source field and step
+// (CSC spmv into a zeroed grid buffer, then a whole-grid muladd: src/fdtd/source.jl:61-177)
+// with one thread per touched cell.  Each thread sums its contributions in ascending
+// source order, which is the accumulation order of SparseArrays' CSC mul!, then applies the
+// reference's update expression, so values are identical.
+//
+// Sampling (src/fdtd/receiver.jl:3-14, transpose(W)*field) runs one lane per receiver and
+// sums its 4 (2-D) or 8 (3-D) weighted samples in ascending index order, like the
+// column-wise dot product of SparseArrays.
+#include "gpi_internal.h"
+
+namespace gpi {
+
+namespace {
+
+enum InjectKind { INJ_P = 0, INJ_NORMAL2 = 1, INJ_NORMAL3 = 2, INJ_VEL = 3 };
+
+struct InjectParams {
+    int kind;
+    int ntarget;
+    int npts;
+    int64_t it;
+    const int64_t* toff;
+    const int32_t* tptr;
+    const int32_t* tsrc;
+    const float* tw;
+    const float* wav;
+    float* f0;
+    float* f1;
+    float* f2;
+    const float* coef;     // dtK or dtM on the stress grid
+    const float* rho;      // velocity sources: density (mod[:rho])
+    const int64_t* roff;   // velocity sources: two rho offsets per target
+    float dt;
+};
+
+__global__ __launch_bounds__(256) void inject_kernel(InjectParams P)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.ntarget) return;
+    const float* w = P.wav + P.it * (int64_t)P.npts;
+    float acc = 0.0f;
+    for (int c = P.tptr[t]; c < P.tptr[t + 1]; ++c) acc += P.tw[c] * w[P.tsrc[c]];
+    const int64_t o = P.toff[t];
+    switch (P.kind) {
+    case INJ_P:
+        P.f0[o] = P.f0[o] + (acc * P.coef[o]);
+        break;
+    case INJ_NORMAL3:
+        P.f2[o] = P.f2[o] + (acc * P.coef[o]);
+        [[fallthrough]];
+    case INJ_NORMAL2:
+        P.f0[o] = P.f0[o] + (acc * P.coef[o]);
+        P.f1[o] = P.f1[o] + (acc * P.coef[o]);
+        break;
+    default: {
+        const float av = (P.rho[P.roff[2 * t]] + P.rho[P.roff[2 * t + 1]]) * 0.5f;
+        P.f0[o] = P.f0[o] + ((acc / av) * P.dt);
+    }
+    }
+}
+
+struct GatherParams {
+    int nset;
+    int nnz;
+    int64_t it;
+    int npts[4];
+    const float* field[4];
+    const int64_t* idx[4];
+    const float* w[4];
+    float* rec[4];
+};
+
+__global__ __launch_bounds__(256) void gather_kernel(GatherParams P)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (s >= P.nset || r >= P.npts[s]) return;
+    const int64_t* ix = P.idx[s] + (int64_t)r * P.nnz;
+    const float* ww = P.w[s] + (int64_t)r * P.nnz;
+    const float* F = P.field[s];
+    float tmp = 0.0f;
+    for (int k = 0; k < P.nnz; ++k) tmp += ww[k] * F[ix[k]];
+    P.rec[s][P.it * (int64_t)P.npts[s] + r] = tmp;
+}
+
+}  // namespace
+
+void inject(const Model& m, const DevPoints& pts, int64_t it, hipStream_t s)
+{
+    if (pts.ntarget == 0) return;
+    InjectParams P{};
+    P.ntarget = pts.ntarget;
+    P.npts = pts.npts;
+    P.it = it;
+    P.toff = pts.toff;
+    P.tptr = pts.tptr;
+    P.tsrc = pts.tsrc;
+    P.tw = pts.tw;
+    P.wav = pts.wav;
+    P.dt = m.g.dt;
+    const int f = pts.field;
+    if (f == GPI_VX || f == GPI_VY || f == GPI_VZ) {
+        P.kind = INJ_VEL;
+        P.f0 = m.field[f];
+        P.rho = m.coef[GPI_C_RHO];
+        P.roff = pts.toff + pts.ntarget;   // rho offsets stored after the target offsets
+    } else if (m.g.physics == 0) {
+        P.kind = INJ_P;
+        P.f0 = m.field[GPI_P];
+        P.coef = m.coef[GPI_C_DTK];
+    } else {
+        P.kind = m.g.ndim == 3 ? INJ_NORMAL3 : INJ_NORMAL2;
+        P.f0 = m.field[GPI_TAUXX];
+        P.f1 = m.field[GPI_TAUZZ];
+        P.f2 = m.g.ndim == 3 ? m.field[GPI_TAUYY] : nullptr;
+        P.coef = m.coef[GPI_C_DTM];
+    }
+    hipLaunchKernelGGL(inject_kernel, dim3((pts.ntarget + 255) / 256), dim3(256), 0, s, P);
+}
+
+void gather(const Model& m, const DevPoints* pts, int n, int64_t it, bool stress, hipStream_t s)
+{
+    GatherParams P{};
+    int maxn = 0;
+    for (int q = 0; q < n; ++q) {
+        const int f = pts[q].field;
+        const bool is_stress = !(f == GPI_VX || f == GPI_VY || f == GPI_VZ);
+        if (is_stress != stress) continue;
+        if (P.nset == 4) {
+            P.it = it;
+            hipLaunchKernelGGL(gather_kernel, dim3((maxn + 255) / 256, P.nset), dim3(256), 0, s, P);
+            P.nset = 0;
+            maxn = 0;
+        }
+        const int k = P.nset++;
+        P.nnz = pts[q].nnz;
+        P.npts[k] = pts[q].npts;
+        P.field[k] = m.field[f];
+        P.idx[k] = pts[q].idx;
+        P.w[k] = pts[q].w;
+        P.rec[k] = pts[q].rec;
+        if (pts[q].npts > maxn) maxn = pts[q].npts;
+    }
+    if (P.nset == 0 || maxn == 0) return;
+    P.it = it;
+    hipLaunchKernelGGL(gather_kernel, dim3((maxn + 255) / 256, P.nset), dim3(256), 0, s, P);
+}
+
+}  // namespace gpi

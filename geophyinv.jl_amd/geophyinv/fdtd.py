@@ -1,0 +1,405 @@
+"""``SeisForwExpt`` / ``update!`` host layer over libgpi.
+
+Mirrors src/fdtd/fdtd.jl:43-295 (construction), src/fdtd/propagate.jl:74-135
+(``update!(pa)``), src/fdtd/medium.jl:128-140 (``update!(pa, medium)``),
+src/fdtd/source.jl:192-246 (``update!(pa, srcwav, src_types)``),
+src/fdtd/ageom.jl:58-98 (``update!(pa, ageom)``) and src/fdtd/getprop.jl:10-31
+(``pa[:data]``).  All host-side preparation happens here exactly as in the reference
+(FP32 coefficients, CPML vectors, spray/interpolation weights); the time loop of every
+supersource runs in libgpi on the GPU.  Julia's ``update!`` is spelled ``update`` here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import warnings
+
+import numpy as np
+
+from . import _lib
+from .cpml import pml_vectors
+from .database import Recs
+from .fields import (COEF_ID, FIELD_ID, coef_grid, derivative_fields, field_shape, get_mgrid,
+                     has_field, wavefields_of)
+from .grid import Grid
+from .media import AcousticMedium, ElasticMedium, Medium, coefficients, padarray
+from .projmat import get_proj_matrix
+from .wavelets import findfreq
+
+F32 = np.float32
+FACES = {"zmin": 0, "zmax": 1, "ymin": 2, "ymax": 3, "xmin": 4, "xmax": 5}
+
+
+class _Attrib:
+    physics = -1
+
+    def __init__(self, mode: str = "forward", npw: int | None = None):
+        if mode not in ("forward", "forward_save", "adjoint"):
+            raise ValueError(f"unknown mode {mode!r}")
+        self.mode = mode
+        self.npw = npw if npw is not None else (1 if mode == "forward" else 2)
+
+    def __repr__(self):
+        return f"{type(self).__name__}(mode={self.mode!r}, npw={self.npw})"
+
+
+class FdtdAcoustic(_Attrib):
+    """Acoustic staggered-grid modelling (src/physics_types.jl:25-49)."""
+    physics = 0
+
+
+class FdtdElastic(_Attrib):
+    """Isotropic elastic staggered-grid modelling (src/physics_types.jl:17-49)."""
+    physics = 1
+
+
+def face_mask(faces, ndim: int) -> int:
+    m = 0
+    for f in faces:
+        if f not in FACES:
+            continue
+        if ndim == 2 and f.startswith("y"):
+            continue
+        m |= 1 << FACES[f]
+    return m
+
+
+def get_source(w: np.ndarray, field: str, src_type: int) -> np.ndarray:
+    """Source wavelet per source type (src/fdtd/source.jl:3-19)."""
+    if src_type == 0:
+        return np.zeros_like(w)
+    if src_type == 1:
+        return w.copy()
+    if src_type == -1:
+        if field not in ("vx", "vy", "vz"):
+            raise ValueError("source type -1 exists for velocity sources only")
+        ww = w * F32(-1)
+        ww = np.roll(ww, -1, axis=0)
+        ww = ww[::-1].copy()
+        ww[0, :] = 0
+        return ww
+    raise ValueError(f"source type {src_type} is not implemented")
+
+
+def check_stability(medium: Medium, dt: float, freqmax: float, order: int, verbose=False):
+    """Courant and sampling checks, warnings only (src/fdtd/stability.jl:8-60)."""
+    H = 20 // order
+    eps = 1.0 / math.sqrt(medium.ndim)
+    ds = [g.step for g in medium.grid]
+    if isinstance(medium, ElasticMedium):
+        nz = medium.vs.m[medium.vs.m != 0]
+        vmin = float(nz.min()) if nz.size else 0.0
+        vmax = math.sqrt(float(medium.vp.bounds[1]) ** 2 + float(medium.vs.bounds[1]) ** 2)
+    else:
+        vmin = float(medium.vp.bounds[0])
+        vmax = float(medium.vp.bounds[1])
+    if freqmax > 0 and math.isfinite(freqmax):
+        ds_temp = round(vmin / H / freqmax, 2)
+        if max(ds) > ds_temp and f"{max(ds):0.2e}" != f"{ds_temp:0.2e}":
+            warnings.warn(f"decrease maximum spatial sampling ({max(ds):0.2e}) below {ds_temp:0.2e}")
+    dt_temp = eps * min(ds) / vmax
+    if dt > dt_temp and f"{dt:0.2e}" != f"{dt_temp:0.2e}":
+        warnings.warn(f"decrease time sampling ({dt:0.2e}) below {dt_temp:0.2e}")
+
+
+class SeisForwExpt:
+    """Forward modelling experiment (``PFdtd``)."""
+
+    def __init__(self, attrib_mod: _Attrib, *, medium: Medium, tgrid: Grid, ageom, srcwav,
+                 rfields=("vz",), pml_faces=None, rigid_faces=None, stressfree_faces=(),
+                 order: int = 2, npml: int | None = None, compat: int = 1, verbose: bool = False,
+                 context: "_lib.Context | None" = None):
+        ndim = medium.ndim
+        if order not in (2, 4, 6, 8):
+            raise ValueError("order must be 2, 4, 6 or 8")
+        if attrib_mod.mode != "forward" or attrib_mod.npw != 1:
+            raise NotImplementedError("only the :forward mode with npw = 1 is available")
+        if isinstance(medium, AcousticMedium) != isinstance(attrib_mod, FdtdAcoustic):
+            raise ValueError("medium type does not match the modelling attribute")
+        physics = attrib_mod.physics
+        if pml_faces is None:
+            pml_faces = ["zmin", "zmax", "ymin", "ymax", "xmax", "xmin"]
+        if rigid_faces is None:
+            rigid_faces = list(pml_faces)
+        if npml is None:
+            npml = 40 + (order - 1)   # _fd_npml (src/GeoPhyInv.jl:90)
+        ageom = list(ageom) if isinstance(ageom, (list, tuple)) else [ageom]
+        srcwav = list(srcwav) if isinstance(srcwav, (list, tuple)) else [srcwav]
+        # ---- checks (src/fdtd/fdtd.jl:88-131)
+        rfields = list(rfields)
+        for rf in rfields:
+            if not has_field(physics, ndim, rf) or rf.startswith("d"):
+                raise ValueError(f"receiver field {rf!r} is not a field of {attrib_mod}")
+        for s in srcwav:
+            for f in s.fields:
+                if not has_field(physics, ndim, f) or f.startswith("d"):
+                    raise ValueError(f"source field {f!r} is not a field of {attrib_mod}")
+        if len(ageom) != len(srcwav):
+            raise ValueError("ageom and srcwav mismatch")
+        if tgrid.last < srcwav[0].grid.last:
+            raise ValueError("modeling time is less than source time")
+        for a, s in zip(ageom, srcwav):
+            if a.ns != s.n:
+                raise ValueError("ageom and srcwav mismatch")
+            if not a.inside(medium.grid):
+                raise ValueError("sources or receivers not inside medium")
+        self.attrib_mod = attrib_mod
+        self.physics = physics
+        self.ndim = ndim
+        self.order = order
+        self.npml = int(npml)
+        self.compat = int(compat)
+        self.tgrid = tgrid
+        self.pml_faces = list(pml_faces)
+        self.rigid_faces = list(dict.fromkeys(list(rigid_faces) + list(pml_faces)))
+        self.stressfree_faces = list(stressfree_faces)
+        self.rfields = rfields
+        self.verbose = verbose
+        self.medium = medium.copy()
+        self.ageom = ageom
+        self.srcwav = [s.copy() for s in srcwav]
+        self.exmedium = padarray(self.medium, self.npml, self.pml_faces)
+        # ---- get_fc (src/fdtd/fdtd.jl:313-332)
+        div = {2: 1.0, 4: 24.0, 6: 1920.0, 8: 107520.0}[order]
+        self.dt32 = F32(tgrid.step)
+        self.dsI = [F32(1.0 / (g.step * div)) for g in self.exmedium.grid]
+        self.coefs = coefficients(self.exmedium, self.dt32, order)
+        # ---- wavelets and frequency bounds (src/fdtd/source.jl:24-58, 192-246)
+        self.src_types = [1]
+        self._fill_wavelets()
+        # ---- CPML (src/fdtd/cpml.jl:144-155): velavg = mean(exmedium[:vp].bounds)
+        b = self.exmedium.vp.bounds
+        self.velavg = F32(F32(b[0] + b[1]) / F32(2))
+        self.pml = pml_vectors(physics, self.exmedium.grid, self.medium.grid, self.pml_faces,
+                               order, self.npml, self.dt32, self.velavg, self.freqpeak)
+        check_stability(self.medium, float(self.dt32), self.freqmax, order, verbose)
+        self._update_weights()
+        self.data = [Recs(a.nr, tgrid, rfields) for a in ageom]
+        self._ctx = context
+        self._model = None
+        self.timing = None
+
+    # ------------------------------------------------------------------ host preparation
+    @property
+    def ext_shape(self) -> tuple:
+        return tuple(g.n for g in self.exmedium.grid)
+
+    def _n3(self):
+        s = self.ext_shape
+        return (s[0], 1, s[1]) if self.ndim == 2 else s
+
+    def _fill_wavelets(self):
+        nt = len(self.tgrid)
+        self.wavelets = []
+        fmins, fmaxs, fpeaks = [], [], []
+        for s in self.srcwav:
+            per = {}
+            peaks = []
+            for f in s.fields:
+                w = get_source(s.d[f], f, self.src_types[0])
+                full = np.zeros((nt, s.n), F32, order="F")
+                snt = min(len(s.grid), nt)
+                full[:snt] = w[:snt]
+                per[f] = full
+                if np.allclose(w, 0.0):
+                    continue
+                fmaxs.append(findfreq(w, s.grid, "max"))
+                fmins.append(findfreq(w, s.grid, "min"))
+                peaks.append(findfreq(w, s.grid, "peak"))
+            self.wavelets.append(per)
+            if peaks:
+                fpeaks.append(float(np.mean(peaks)))
+        self.freqmin = F32(max(fmins) if fmins else 0.0)
+        self.freqmax = F32(min(fmaxs) if fmaxs else math.inf)
+        self.freqpeak = F32(np.mean(fpeaks) if fpeaks else 0.0)
+
+    def _update_weights(self):
+        grid = self.exmedium.grid
+        self.spray = []
+        self.interp = []
+        for a, s in zip(self.ageom, self.srcwav):
+            sp = {}
+            for f in s.fields:
+                sp[f] = get_proj_matrix(a.src_points(), get_mgrid(f, self.physics, grid, self.order), self.compat)
+            self.spray.append(sp)
+            ip = {}
+            for f in self.rfields:
+                ip[f] = get_proj_matrix(a.rec_points(), get_mgrid(f, self.physics, grid, self.order), self.compat)
+            self.interp.append(ip)
+
+    def grid_struct(self) -> "_lib.gpi_grid":
+        g = _lib.gpi_grid()
+        g.ndim = self.ndim
+        g.order = self.order
+        g.physics = self.physics
+        g.npml = self.npml
+        n3 = self._n3()
+        for a in range(3):
+            g.n[a] = n3[a]
+        g.pml_mask = face_mask(self.pml_faces, self.ndim)
+        g.rigid_mask = face_mask(self.rigid_faces, self.ndim)
+        g.free_mask = face_mask(self.stressfree_faces, self.ndim)
+        g.dt = float(self.dt32)
+        ds = [self.dsI[0], self.dsI[1] if self.ndim == 3 else F32(0), self.dsI[-1]]
+        for a in range(3):
+            g.dsI[a] = float(ds[a])
+        return g
+
+    def coef_arrays(self):
+        keep, arrs = [], []
+        for name, a in self.coefs.items():
+            if coef_grid(self.physics, self.ndim, name) is None:
+                continue
+            a = np.asfortranarray(a, dtype=F32)
+            keep.append(a)
+            x = _lib.gpi_array()
+            x.id = COEF_ID[name]
+            x.count = a.size
+            x.data = _lib.fptr(a)
+            arrs.append(x)
+        return keep, (_lib.gpi_array * len(arrs))(*arrs)
+
+    # ------------------------------------------------------------------ device side
+    def _ensure_model(self):
+        if self._model is not None:
+            return
+        if self._ctx is None:
+            self._ctx = _lib.default_context()
+        ctx = self._ctx
+        keep, arrs = self.coef_arrays()
+        pmls = []
+        for df, (a, b, k) in self.pml.items():
+            p = _lib.gpi_pml()
+            p.field = FIELD_ID[df]
+            p.len = a.size
+            p.a, p.b, p.kI = _lib.fptr(a), _lib.fptr(b), _lib.fptr(k)
+            pmls.append(p)
+        parr = (_lib.gpi_pml * len(pmls))(*pmls)
+        h = C.c_uint64()
+        g = self.grid_struct()
+        ctx.check(ctx.lib.gpi_model_create(ctx.h, C.byref(g), arrs, len(arrs), parr, len(pmls), C.byref(h)))
+        self._model = h.value
+
+    def free(self):
+        if self._model is not None and self._ctx is not None and self._ctx.h:
+            self._ctx.check(self._ctx.lib.gpi_model_free(self._ctx.h, self._model))
+        self._model = None
+
+    def run_shot(self, iss: int, final_fields=()):
+        """Propagate supersource ``iss`` on the GPU; returns (records dict, finals dict)."""
+        self._ensure_model()
+        ctx = self._ctx
+        nt = len(self.tgrid)
+        keep = []
+        srcs = []
+        wavs = []
+        for f, (idx, w) in self.spray[iss].items():
+            idx = np.ascontiguousarray(idx)
+            w = np.ascontiguousarray(w)
+            wv = np.asfortranarray(self.wavelets[iss][f], dtype=F32)
+            keep += [idx, w, wv]
+            p = _lib.gpi_points()
+            p.field, p.npts, p.nnz = FIELD_ID[f], idx.shape[0], idx.shape[1]
+            p.idx, p.w = _lib.iptr(idx), _lib.fptr(w)
+            srcs.append(p)
+            wavs.append(_lib.fptr(wv))
+        recs, outs, names = [], [], []
+        for f, (idx, w) in self.interp[iss].items():
+            idx = np.ascontiguousarray(idx)
+            w = np.ascontiguousarray(w)
+            out = np.zeros((nt, idx.shape[0]), F32, order="F")
+            keep += [idx, w, out]
+            p = _lib.gpi_points()
+            p.field, p.npts, p.nnz = FIELD_ID[f], idx.shape[0], idx.shape[1]
+            p.idx, p.w = _lib.iptr(idx), _lib.fptr(w)
+            recs.append(p)
+            outs.append(out)
+            names.append(f)
+        finals = {}
+        fptrs = []
+        for f in final_fields:
+            a = np.zeros(field_shape(self.physics, self.ndim, self.order, self.ext_shape, f), F32, order="F")
+            finals[f] = a
+            fptrs.append(_lib.fptr(a))
+        shot = _lib.gpi_shot()
+        shot.nt = nt
+        shot.nsrc = len(srcs)
+        shot.nrec = len(recs)
+        shot.src = (_lib.gpi_points * max(1, len(srcs)))(*srcs)
+        shot.wavelets = (C.POINTER(C.c_float) * max(1, len(wavs)))(*wavs)
+        shot.rec = (_lib.gpi_points * max(1, len(recs)))(*recs)
+        shot.records = (C.POINTER(C.c_float) * max(1, len(outs)))(*[_lib.fptr(o) for o in outs])
+        fids = np.array([FIELD_ID[f] for f in final_fields], np.int32)
+        shot.nfinal = len(final_fields)
+        shot.final_fields = fids.ctypes.data_as(C.POINTER(C.c_int32))
+        shot.final_out = (C.POINTER(C.c_float) * max(1, len(fptrs)))(*fptrs)
+        ctx.check(ctx.lib.gpi_forward(ctx.h, self._model, C.byref(shot)))
+        self.timing = ctx.timing()
+        return dict(zip(names, outs)), finals
+
+    def update(self):
+        """``update!(pa)``: model every supersource and fill ``pa["data"]``."""
+        for iss in range(len(self.ageom)):
+            recs, _ = self.run_shot(iss)
+            for f, r in recs.items():
+                self.data[iss].d[f][:, :] = r
+        return self.timing
+
+    def update_medium(self, medium: Medium):
+        """``update!(pa, medium)`` without reallocation (src/fdtd/medium.jl:128-140)."""
+        if type(medium) is not type(self.medium) or [g.n for g in medium.grid] != [g.n for g in self.medium.grid]:
+            raise ValueError("medium must match the experiment's medium type and grid")
+        self.medium = medium.copy()
+        self.exmedium = padarray(self.medium, self.npml, self.pml_faces)
+        self.coefs = coefficients(self.exmedium, self.dt32, self.order)
+        if self._model is not None:
+            keep, arrs = self.coef_arrays()
+            self._ctx.check(self._ctx.lib.gpi_model_update(self._ctx.h, self._model, arrs, len(arrs)))
+
+    def update_srcwav(self, srcwav, src_types=None):
+        """``update!(pa, srcwav, src_types)`` (src/fdtd/source.jl:192-246)."""
+        srcwav = list(srcwav) if isinstance(srcwav, (list, tuple)) else [srcwav]
+        if len(srcwav) != len(self.srcwav):
+            raise ValueError("srcwav must have one entry per supersource")
+        self.srcwav = [s.copy() for s in srcwav]
+        if src_types is not None:
+            self.src_types = list(src_types) if isinstance(src_types, (list, tuple)) else [src_types]
+        self._fill_wavelets()
+        self._update_weights()
+
+    def update_ageom(self, ageom):
+        """``update!(pa, ageom)`` (src/fdtd/ageom.jl:58-98)."""
+        ageom = list(ageom) if isinstance(ageom, (list, tuple)) else [ageom]
+        for a, b in zip(ageom, self.ageom):
+            if a.ns != b.ns or a.nr != b.nr:
+                raise ValueError("ageom must keep the number of sources and receivers")
+            if not a.inside(self.medium.grid):
+                raise ValueError("sources or receivers not inside medium")
+        self.ageom = ageom
+        self._update_weights()
+
+    def __getitem__(self, key):
+        """``pa[:data]`` and friends (src/fdtd/getprop.jl:10-31)."""
+        if isinstance(key, tuple):
+            key, iss = key
+        else:
+            iss = None
+        if key == "data":
+            return self.data if iss is None else self.data[iss]
+        if key in ("ageom", "srcwav", "medium", "exmedium", "rfields", "pml_faces", "rigid_faces"):
+            return getattr(self, key)
+        raise KeyError(key)
+
+
+def update(obj, *args, **kw):
+    """Dispatch of the reference's ``update!`` methods on a ``SeisForwExpt``."""
+    if not isinstance(obj, SeisForwExpt):
+        raise TypeError("update expects a SeisForwExpt")
+    if not args:
+        return obj.update()
+    a = args[0]
+    if isinstance(a, Medium):
+        return obj.update_medium(a)
+    if isinstance(a, list) and a and hasattr(a[0], "ns") and hasattr(a[0], "s"):
+        return obj.update_ageom(a)
+    return obj.update_srcwav(a, *args[1:], **kw)

@@ -1,0 +1,78 @@
+"""Synthetic workloads of the FDTD engine, generated from fixed seeds (numpy PCG64).
+
+``acoustic_anomaly_2d`` is the 201x201 acoustic single-shot case and ``elastic_layers_2d``
+the 2001x2001 P-SV drop-in case at 2.5 m; both build (medium, ageom, srcwav, tgrid, kw)
+ready for ``SeisForwExpt``.  ``scale`` shrinks a workload for quick checks.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .database import AGeomss, Srcs
+from .grid import Grid
+from .media import AcousticMedium, ElasticMedium
+from .wavelets import ricker
+
+
+def _box_smooth(a: np.ndarray, k: int) -> np.ndarray:
+    """k x k moving average with edge replication."""
+    r = k // 2
+    p = np.pad(a, r, mode="edge")
+    c = np.cumsum(np.cumsum(p, axis=0), axis=1)
+    c = np.pad(c, ((1, 0), (1, 0)))
+    s = c[k:, k:] - c[:-k, k:] - c[k:, :-k] + c[:-k, :-k]
+    return s / (k * k)
+
+
+def acoustic_anomaly_2d(n: int = 201, ds: float = 5.0, nt: int = 1000, npml: int = 20,
+                        freq: float = 15.0, nrec: int = 101):
+    """2-D acoustic, vp = 2000 m/s with a +10% Gaussian anomaly (radius 50 m) at the centre,
+    rho = 2000, Ricker on p five cells below the surface at mid-x, receivers along the
+    surface recording p, vx, vz, dt at CFL 0.5."""
+    L = (n - 1) * ds
+    grid = [Grid.range(0.0, L, length=n), Grid.range(0.0, L, length=n)]
+    z = grid[0].values()[:, None]
+    x = grid[1].values()[None, :]
+    cz, cx = L / 2, L / 2
+    r2 = (z - cz) ** 2 + (x - cx) ** 2
+    vp = (2000.0 * (1.0 + 0.10 * np.exp(-r2 / (2.0 * 50.0 ** 2)))).astype(np.float32)
+    rho = np.full((n, n), 2000.0, np.float32)
+    medium = AcousticMedium(grid, vp, rho)
+    vmax = 2200.0
+    dt = 0.5 * ds / vmax
+    tgrid = Grid(0.0, dt, nt)
+    sx = L / 2
+    ageom = [AGeomss({"z": [5 * ds], "x": [sx]},
+                     {"z": np.zeros(nrec), "x": np.linspace(0.0, L, nrec)})]
+    src = Srcs(1, tgrid, ["p"])
+    src.update(["p"], ricker(freq, tgrid))
+    kw = dict(rfields=["p", "vx", "vz"], npml=npml)
+    return medium, ageom, [src], tgrid, kw
+
+
+def elastic_layers_2d(n: int = 2001, ds: float = 2.5, nt: int = 4000, npml: int = 40,
+                      freq: float = 20.0, nrec: int = 801, nlayers: int = 8, seed: int = 2002):
+    """2-D P-SV: equal-thickness flat layers vp 1500->4500 m/s, vs = vp/1.8, Gardner density,
+    a +-3% white-noise perturbation smoothed over 5 cells, explosive (tauxx) Ricker source,
+    surface receivers recording vx and vz, dt at CFL 0.5 of sqrt(vp^2 + vs^2)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    L = (n - 1) * ds
+    grid = [Grid.range(0.0, L, length=n), Grid.range(0.0, L, length=n)]
+    layer = np.minimum((np.arange(n) * nlayers) // n, nlayers - 1)
+    vp1 = 1500.0 + layer * (3000.0 / (nlayers - 1))
+    vp = np.repeat(vp1[:, None], n, axis=1)
+    eps = _box_smooth(rng.uniform(-0.03, 0.03, size=(n, n)), 5)
+    vp = vp * (1.0 + eps)
+    vs = vp / 1.8
+    rho = 310.0 * vp ** 0.25
+    medium = ElasticMedium(grid, vp.astype(np.float32), vs.astype(np.float32), rho.astype(np.float32))
+    dt = 0.5 * ds / math.sqrt(4500.0 ** 2 + 2500.0 ** 2)
+    tgrid = Grid(0.0, dt, nt)
+    ageom = [AGeomss({"z": [10 * ds], "x": [L / 2]},
+                     {"z": np.zeros(nrec), "x": np.linspace(0.0, L, nrec)})]
+    src = Srcs(1, tgrid, ["tauxx"])
+    src.update(["tauxx"], ricker(freq, tgrid))
+    kw = dict(rfields=["vx", "vz"], npml=npml)
+    return medium, ageom, [src], tgrid, kw

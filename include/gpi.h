@@ -1,0 +1,193 @@
+/*
+ * libgpi — MI355X (gfx950) staggered-grid velocity-stress FDTD engine, C ABI.
+ *
+ * This is the drop-in boundary for the time-stepping body of GeoPhyInv.jl's
+ * `mod_x_proc!` (src/fdtd/propagate.jl:138-261).  The host (Julia `ccall`, Python
+ * `ctypes`, C/C++) keeps building the FP32 coefficient arrays (src/fdtd/medium.jl:143-221),
+ * the CPML vectors (src/fdtd/cpml.jl:8-142) and the spray/interpolation weights
+ * (src/proj_mat.jl:208-247, src/fdtd/ageom.jl:29-54) exactly as the reference does, and
+ * hands them to this library, which keeps them resident in HBM and runs the time loop.
+ *
+ * Conventions
+ *  - All arrays are FP32, column-major with z fastest (Julia layout): a field of size
+ *    (nz, ny, nx) has element (iz, iy, ix) at iz + nz*(iy + ny*ix), zero-based.  2-D grids
+ *    use ny = 1 and omit the y axis everywhere.
+ *  - Every entry point returns 0 on success or a negative GPI_E* code; a message is then
+ *    available from gpi_last_error().  No entry point aborts or throws across the ABI.
+ *  - A context is not re-entrant: use one per host thread or serialise calls.
+ *  - The caller owns every host pointer for the duration of a call; the library copies
+ *    what it keeps.
+ */
+#ifndef GPI_H
+#define GPI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPI_ABI_VERSION 1
+
+/* error codes */
+#define GPI_OK 0
+#define GPI_EINVAL (-1)      /* bad argument or inconsistent sizes */
+#define GPI_ENOMEM (-2)      /* device allocation failed or over the context's HBM budget */
+#define GPI_EHIP (-3)        /* HIP runtime error */
+#define GPI_ENONFINITE (-4)  /* non-finite values found in the records */
+#define GPI_ENOTFOUND (-5)   /* unknown handle */
+
+/* Field ids.  Names and staggered grids follow src/fields.jl:9-42 (types) and
+ * src/fields.jl:92-671 (grids).  The derivative fields exist for the CPML coefficient
+ * vectors (src/fdtd/cpml.jl:109-120); the kernels never store them. */
+enum gpi_field {
+    GPI_P = 0, GPI_VX, GPI_VY, GPI_VZ,
+    GPI_TAUXX, GPI_TAUYY, GPI_TAUZZ, GPI_TAUXY, GPI_TAUXZ, GPI_TAUYZ,
+    GPI_DPDX, GPI_DPDY, GPI_DPDZ,
+    GPI_DVXDX, GPI_DVYDY, GPI_DVZDZ, GPI_DVXDY, GPI_DVXDZ, GPI_DVYDX, GPI_DVYDZ, GPI_DVZDX,
+    GPI_DVZDY,
+    GPI_DTAUXXDX, GPI_DTAUYYDY, GPI_DTAUZZDZ, GPI_DTAUXYDX, GPI_DTAUXYDY, GPI_DTAUXZDX,
+    GPI_DTAUXZDZ, GPI_DTAUYZDY, GPI_DTAUYZDZ,
+    GPI_NFIELDS
+};
+
+/* Coefficient ids: the reference's `dmod` (dependent) and `mod` (independent) arrays,
+ * src/fdtd/medium.jl:81-95.  Each lives on the grid of the field named in that table:
+ *   DTINVAV{X,Y,Z}IRHO on dpdx/dpdy/dpdz (acoustic) or dtauxxdx/dtauyydy/dtauzzdz (elastic);
+ *   DTK, DTM, DTLAMBDA, RHO, INVK, INVLAMBDA, INVMU on the stress grid;
+ *   DTAVMU (2-D) on dvxdz; DTAVXZIMU on dvxdz, DTAVXYIMU on dvxdy, DTAVYZIMU on dvydz. */
+enum gpi_coef {
+    GPI_C_DTINVAVXIRHO = 0, GPI_C_DTINVAVYIRHO, GPI_C_DTINVAVZIRHO, GPI_C_DTK, GPI_C_DTM,
+    GPI_C_DTLAMBDA, GPI_C_DTAVMU, GPI_C_DTAVXZIMU, GPI_C_DTAVXYIMU, GPI_C_DTAVYZIMU,
+    GPI_C_RHO, GPI_C_INVK, GPI_C_INVLAMBDA, GPI_C_INVMU,
+    GPI_NCOEF
+};
+
+/* Face bits: bit (2*axis + side), axis 0 = z, 1 = y, 2 = x; side 0 = min, 1 = max.
+ * Mirrors the reference's pml_faces / rigid_faces / stressfree_faces symbol lists
+ * (src/fdtd/fdtd.jl:61-78, :215). */
+#define GPI_FACE(axis, side) (1u << (2 * (axis) + (side)))
+
+typedef struct gpi_ctx gpi_ctx;
+
+/* Grid and scheme of one modelling problem (P_common's ic/fc, src/fdtd/fdtd.jl:201-202,
+ * src/fdtd/fdtd.jl:301-332). */
+typedef struct gpi_grid {
+    int32_t ndim;        /* 2 or 3 (LocalPreferences `ndims`) */
+    int32_t order;       /* 2, 4, 6 or 8 (LocalPreferences `order`) */
+    int32_t physics;     /* 0 FdtdAcoustic, 1 FdtdElastic */
+    int32_t npml;        /* CPML points per face; the reference uses 40 + order - 1 */
+    int64_t n[3];        /* extended stress grid nz, ny, nx (ny = 1 in 2-D) */
+    uint32_t pml_mask;   /* faces carrying CPML memory updates */
+    uint32_t rigid_mask; /* faces with Dirichlet velocity conditions (pml ∪ rigid) */
+    uint32_t free_mask;  /* stress-free faces (only zmin is implemented by the reference) */
+    uint32_t reserved;
+    float dt;            /* fc[:dt] as Float32 */
+    float dsI[3];        /* fc[:dzI], fc[:dyI], fc[:dxI] as Float32, order divisor included */
+} gpi_grid;
+
+/* One coefficient array handed over at model creation. */
+typedef struct gpi_array {
+    int32_t id;          /* gpi_coef */
+    int32_t reserved;
+    int64_t count;       /* number of floats; must equal the size of its grid */
+    const float* data;   /* host pointer, column-major */
+} gpi_array;
+
+/* CPML vectors of one derivative field (pml[df][:a], [:b], [:kI], src/fdtd/cpml.jl:88-102):
+ * length 2*npml, the first npml for the min face and the last npml for the max face. */
+typedef struct gpi_pml {
+    int32_t field;       /* derivative field id */
+    int32_t len;         /* 2*npml */
+    const float* a;
+    const float* b;
+    const float* kI;
+} gpi_pml;
+
+/* Sparse spray or interpolation weights of one field (one CSC matrix of
+ * src/proj_mat.jl:229-247): npts columns with nnz entries each, entries of a column in
+ * ascending row order, row = zero-based column-major index into the field's array. */
+typedef struct gpi_points {
+    int32_t field;       /* field the points live on */
+    int32_t npts;        /* sources (spray) or receivers (interpolation) */
+    int32_t nnz;         /* 4 in 2-D, 8 in 3-D */
+    int32_t reserved;
+    const int64_t* idx;  /* npts*nnz */
+    const float* w;      /* npts*nnz */
+} gpi_points;
+
+/* One supersource ("shot") of a forward run, src/fdtd/propagate.jl:138-261 with
+ * npw = 1 and mode :forward. */
+typedef struct gpi_shot {
+    int64_t nt;                      /* time steps (length(tgrid)) */
+    int32_t nsrc;                    /* source fields */
+    int32_t nrec;                    /* receiver fields */
+    const gpi_points* src;           /* nsrc spray matrices (pap.ss[i].ssprayw) */
+    const float* const* wavelets;    /* nsrc arrays of nt x npts, time fastest (Srcs.d) */
+    const gpi_points* rec;           /* nrec interpolation matrices (rinterpolatew) */
+    float* const* records;           /* nrec host outputs of nt x npts, time fastest (Recs.d) */
+    int32_t nfinal;                  /* optional final wavefields to copy back */
+    int32_t reserved;
+    const int32_t* final_fields;     /* field ids */
+    float* const* final_out;         /* host outputs, each of its field's full size */
+} gpi_shot;
+
+/* Timing of the last gpi_forward call on a context (device time, HIP events on the
+ * context's stream). */
+typedef struct gpi_timing {
+    double total_ms;        /* whole time loop, inputs already resident */
+    double velocity_avg_ms; /* mean duration of one velocity-pass launch (sampled steps) */
+    double stress_avg_ms;   /* mean duration of one stress-pass launch (sampled steps) */
+    int64_t steps;          /* time steps run */
+    int64_t sampled;        /* steps whose passes were bracketed by events (0 = sampling off) */
+    int64_t launches;       /* kernel launches issued by the time loop */
+} gpi_timing;
+
+/* ---- context ------------------------------------------------------------------------ */
+/* No reference counterpart: the reference selects its backend at load time
+ * (src/GeoPhyInv.jl:95-100).  hbm_budget_bytes = 0 means "free HBM at creation". */
+int gpi_create(gpi_ctx** out, int device, uint64_t hbm_budget_bytes);
+void gpi_destroy(gpi_ctx* ctx);
+const char* gpi_last_error(const gpi_ctx* ctx);
+int gpi_abi_version(void);
+
+/* Size of a field's array for a grid (src/fields.jl:48-86 `zeros(::field, ...)`).
+ * Returns GPI_EINVAL when the field does not exist for that physics/ndim. */
+int gpi_field_shape(int32_t ndim, int32_t order, int32_t physics, const int64_t n[3],
+                    int32_t field, int64_t out[3]);
+
+/* ---- model (medium + scheme resident in HBM) ----------------------------------------- */
+/* Replaces the device-side state built by PFdtd (src/fdtd/fdtd.jl:135-160) and refreshed
+ * by update!(pa, medium) (src/fdtd/medium.jl:128-140): coefficient arrays and CPML
+ * vectors.  Required coefficients: acoustic DTINVAV{X,(Y),Z}IRHO, DTK, RHO; elastic
+ * DTINVAV{X,(Y),Z}IRHO, DTM, DTLAMBDA, DTAVMU (2-D) or DTAV{XZ,XY,YZ}IMU (3-D), RHO.
+ * One gpi_pml per derivative field of the physics when pml_mask != 0. */
+int gpi_model_create(gpi_ctx* ctx, const gpi_grid* grid, const gpi_array* coefs,
+                     int32_t ncoefs, const gpi_pml* pml, int32_t npml_arrays,
+                     uint64_t* model_out);
+/* update!(pa, medium) without reallocation: overwrite coefficient arrays in place. */
+int gpi_model_update(gpi_ctx* ctx, uint64_t model, const gpi_array* coefs, int32_t ncoefs);
+int gpi_model_free(gpi_ctx* ctx, uint64_t model);
+
+/* ---- forward modelling ---------------------------------------------------------------- */
+/* One supersource of update!(pa) in :forward mode (src/fdtd/propagate.jl:74-135 with the
+ * body of mod_x_proc!, :138-261): zero the wavefields and CPML memory, run nt steps
+ *   record p; velocity pass (stress derivatives + CPML + velocity update + rigid faces);
+ *   velocity sources; record v; stress pass (velocity derivatives + CPML + stress update +
+ *   free surface); stress sources,
+ * then the trailing velocity half step of :254-255, and copy records (and the optional
+ * final wavefields) to the host. */
+int gpi_forward(gpi_ctx* ctx, uint64_t model, const gpi_shot* shot);
+
+/* Device timing of the last gpi_forward on this context. */
+int gpi_last_timing(const gpi_ctx* ctx, gpi_timing* out);
+
+/* Bracket the velocity and stress passes of up to `nsamples` evenly spaced steps of each
+ * following gpi_forward with HIP events (0 turns sampling off).  No reference counterpart:
+ * it replaces the TimerOutputs sections of src/fdtd/propagate.jl:176-232 with device
+ * time. */
+int gpi_set_pass_sampling(gpi_ctx* ctx, int32_t nsamples);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
