@@ -1,61 +1,32 @@
 // 2-D staggered-grid velocity-stress kernels for gfx950.
 //
 // One launch per half step.  The velocity pass fuses what the reference runs as
-// compute_dp!/compute_dstress!, the CPML memory strips, compute_v! and the rigid-face
-// kernels (src/fdtd/advance_acou.jl:11-58, src/fdtd/advance_elastic.jl:534-608,
-// src/fdtd/cpml.jl:158-215, src/fdtd/dirichlet.jl:3-78); the stress pass fuses
-// compute_dv!, the CPML strips, compute_p!/compute_stressii!/compute_stressij! and the
-// free surface (src/fdtd/advance_acou.jl:59-96, src/fdtd/advance_elastic.jl:609-682).
+// compute_dp!/compute_dstress!, the CPML memory strips, compute_v!, the rigid-face kernels
+// and add_velocity_source! (src/fdtd/advance_acou.jl:11-58, src/fdtd/advance_elastic.jl:
+// 534-608, src/fdtd/cpml.jl:158-215, src/fdtd/dirichlet.jl:3-78, src/fdtd/source.jl:124-157);
+// its extra workgroups record p (src/fdtd/propagate.jl:177).  The stress pass fuses
+// compute_dv!, the CPML strips, compute_p!/compute_stressii!/compute_stressij!, the free
+// surface and add_stress_source! (src/fdtd/advance_acou.jl:59-96, src/fdtd/advance_elastic.jl:
+// 609-682, src/fdtd/source.jl:61-119); its extra workgroups record v (propagate.jl:208).
 // Derivatives never leave registers.  Arithmetic keeps the reference's FP32 evaluation
-// order (the library is built with -ffp-contract=off), so results match the unfused
-// CPU restatement bit for bit.
+// order (the library is built with -ffp-contract=off), so results match the unfused CPU
+// restatement bit for bit.
 //
 // Layout: every wavefield and coefficient array of a model uses the same column pitch
-// `ld` (floats), z contiguous.  A thread owns one derivative-grid point (i, j); lanes of
-// a wavefront run along z so every load and store is a coalesced 256-byte row segment.
-#include "gpi_internal.h"
+// `ld` (floats), z contiguous.  A thread owns one derivative-grid point (i, j); lanes of a
+// wavefront run along z so every load and store is a coalesced 256-byte row segment.
+#include "fdtd2d.h"
+#include "pass_extras.h"
 #include "stencil.h"
 
 namespace gpi {
 
 namespace {
 
-struct VelParams2D {
-    const float* s0;   // acoustic p | elastic tauxx
-    const float* s1;   // elastic tauzz
-    const float* s2;   // elastic tauxz
-    float* vx;
-    float* vz;
-    const float* cx;   // dtinvavxirho
-    const float* cz;   // dtinvavzirho
-    Pml qx0, qx1;      // x-derivative CPML: acoustic dpdx | elastic dtauxxdx, dtauxzdx
-    Pml qz0, qz1;      // z-derivative CPML: acoustic dpdz | elastic dtauzzdz, dtauxzdz
-    int nz, nx, ld, npml;
-    uint32_t pml_mask, rigid_mask;
-    float dxI, dzI;
-};
-
-struct StrParams2D {
-    const float* vx;
-    const float* vz;
-    float* s0;         // acoustic p | elastic tauxx
-    float* s1;         // elastic tauzz
-    float* s2;         // elastic tauxz
-    const float* c0;   // acoustic dtK | elastic dtM
-    const float* c1;   // elastic dtlambda
-    const float* c2;   // elastic dtavmu
-    Pml qx0, qx1;      // acoustic dvxdx | elastic dvxdx, dvzdx
-    Pml qz0, qz1;      // acoustic dvzdz | elastic dvzdz, dvxdz
-    int nz, nx, ld, npml;
-    uint32_t pml_mask, free_mask;
-    float dxI, dzI;
-};
-
-// CPML memory update of cpml.jl:178-180 on a derivative that varies along x (column j of
-// a derivative grid with q.nd columns).  Min face first, then max face, as the reference
+// CPML memory update of cpml.jl:178-180 on a derivative that varies along x (column j of a
+// derivative grid with q.nd columns).  Min face first, then max face, as the reference
 // launches memorynpx! for xmin before xmax.
-__device__ __forceinline__ float cpml_x(float d, const Pml& q, int i, int j, int npml,
-                                        uint32_t mask)
+__device__ __forceinline__ float cpml_x(float d, const Pml& q, int i, int j, int npml, uint32_t mask)
 {
     if ((mask & GPI_FACE(2, 0)) && j < npml) {
         float* m = q.psi + i + q.ps2 * j;
@@ -74,8 +45,7 @@ __device__ __forceinline__ float cpml_x(float d, const Pml& q, int i, int j, int
 }
 
 // Same along z (row i of a derivative grid with q.nd rows); psi rows are the 2*npml strip.
-__device__ __forceinline__ float cpml_z(float d, const Pml& q, int i, int j, int npml,
-                                        uint32_t mask)
+__device__ __forceinline__ float cpml_z(float d, const Pml& q, int i, int j, int npml, uint32_t mask)
 {
     if ((mask & GPI_FACE(0, 0)) && i < npml) {
         float* m = q.psi + i + q.ps2 * j;
@@ -116,43 +86,55 @@ __device__ __forceinline__ float dx_at(const float* a, int off, int ld)
 // ---------------------------------------------------------------------------------------
 
 template <int O>
-__global__ __launch_bounds__(256) void vel2d_acoustic(VelParams2D P)
+__global__ __launch_bounds__(256) void vel2d_acoustic(VelParams2D P, PassExtras X)
 {
     constexpr int H = O - 1;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int b = blockIdx.x;
+    const int nbm = P.gx * P.gy;
+    if (b >= nbm) {
+        gather_block(X, b - nbm);
+        return;
+    }
+    const int i = (b % P.gx) * 64 + threadIdx.x;
+    const int j = (b / P.gx) * 4 + threadIdx.y;
     const int ld = P.ld;
     // vx[inn] += dtinvavxirho * dpdx, dpdx = D_xi(p) * dxI on (nz-2h) x (nx-h)
     if (i < P.nz - 2 * H && j < P.nx - H) {
         float d = dx_at<O>(P.s0, (i + H) + ld * j, ld) * P.dxI;
         d = cpml_x(d, P.qx0, i, j, P.npml, P.pml_mask);
         const int o = (i + H) + ld * (j + H);
-        const float nv = P.vx[o] + P.cx[i + ld * j] * d;
-        P.vx[o] = nv;
-        // rigid x faces: the only nonzero ghost is the mirror of the first/last interior
-        // column (dirichlet.jl ighostmin/ighostmax with ifd = 1)
+        float nv = P.vx[o] + P.cx[i + ld * j] * d;
+        // rigid x faces: the only nonzero ghost mirrors the first/last interior column
+        // (dirichlet.jl ighostmin/ighostmax with ifd = 1), taken before the source term
         if ((P.rigid_mask & GPI_FACE(2, 0)) && j == 0) P.vx[(i + H)] = -nv;
-        if ((P.rigid_mask & GPI_FACE(2, 1)) && j == P.nx - H - 1)
-            P.vx[(i + H) + ld * (P.nx + H - 1)] = -nv;
+        if ((P.rigid_mask & GPI_FACE(2, 1)) && j == P.nx - H - 1) P.vx[(i + H) + ld * (P.nx + H - 1)] = -nv;
+        if (X.ninj) nv = inject_velocity(X, INJ_VX, b, o, nv, P.rho, P.dt);
+        P.vx[o] = nv;
     }
     // vz[inn] += dtinvavzirho * dpdz, dpdz = D_zi(p) * dzI on (nz-h) x (nx-2h)
     if (i < P.nz - H && j < P.nx - 2 * H) {
         float d = dz_at<O>(P.s0, i + ld * (j + H)) * P.dzI;
         d = cpml_z(d, P.qz0, i, j, P.npml, P.pml_mask);
         const int o = (i + H) + ld * (j + H);
-        const float nv = P.vz[o] + P.cz[i + ld * j] * d;
-        P.vz[o] = nv;
+        float nv = P.vz[o] + P.cz[i + ld * j] * d;
         if ((P.rigid_mask & GPI_FACE(0, 0)) && i == 0) P.vz[ld * (j + H)] = -nv;
-        if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1)
-            P.vz[(P.nz + H - 1) + ld * (j + H)] = -nv;
+        if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1) P.vz[(P.nz + H - 1) + ld * (j + H)] = -nv;
+        if (X.ninj) nv = inject_velocity(X, INJ_VZ, b, o, nv, P.rho, P.dt);
+        P.vz[o] = nv;
     }
 }
 
 template <int O>
-__global__ __launch_bounds__(256) void str2d_acoustic(StrParams2D P)
+__global__ __launch_bounds__(256) void str2d_acoustic(StrParams2D P, PassExtras X)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int b = blockIdx.x;
+    const int nbm = P.gx * P.gy;
+    if (b >= nbm) {
+        gather_block(X, b - nbm);
+        return;
+    }
+    const int i = (b % P.gx) * 64 + threadIdx.x;
+    const int j = (b / P.gx) * 4 + threadIdx.y;
     if (i >= P.nz || j >= P.nx) return;
     const int ld = P.ld;
     const int o = i + ld * j;
@@ -160,7 +142,15 @@ __global__ __launch_bounds__(256) void str2d_acoustic(StrParams2D P)
     dvx = cpml_x(dvx, P.qx0, i, j, P.npml, P.pml_mask);
     float dvz = dz_at<O>(P.vz, o) * P.dzI;           // D_za(vz)
     dvz = cpml_z(dvz, P.qz0, i, j, P.npml, P.pml_mask);
-    P.s0[o] = P.s0[o] + (dvx + dvz) * P.c0[o];
+    const float c = P.c0[o];
+    float p = P.s0[o] + (dvx + dvz) * c;
+    // p = p + pv * dtK per source field (muladd_tauii!, source.jl:160-163)
+    for (int q = 0; q < X.ninj; ++q) {
+        float acc;
+        int t;
+        if (X.inj[q].kind == INJ_P && injection_sum(X.inj[q], b, o, X.it, acc, t)) p = p + (acc * c);
+    }
+    P.s0[o] = p;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -168,11 +158,17 @@ __global__ __launch_bounds__(256) void str2d_acoustic(StrParams2D P)
 // ---------------------------------------------------------------------------------------
 
 template <int O>
-__global__ __launch_bounds__(256) void vel2d_elastic(VelParams2D P)
+__global__ __launch_bounds__(256) void vel2d_elastic(VelParams2D P, PassExtras X)
 {
     constexpr int H = O - 1;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int b = blockIdx.x;
+    const int nbm = P.gx * P.gy;
+    if (b >= nbm) {
+        gather_block(X, b - nbm);
+        return;
+    }
+    const int i = (b % P.gx) * 64 + threadIdx.x;
+    const int j = (b / P.gx) * 4 + threadIdx.y;
     const int ld = P.ld;
     // vx[inn] -= dtinvavxirho*(dtauxxdx + dtauxzdz) on (nz-2h) x (nx-h)
     if (i < P.nz - 2 * H && j < P.nx - H) {
@@ -181,11 +177,11 @@ __global__ __launch_bounds__(256) void vel2d_elastic(VelParams2D P)
         float d2 = dz_at<O>(P.s2, i + ld * j) * P.dzI;              // D_za(tauxz)
         d2 = cpml_z(d2, P.qz1, i, j, P.npml, P.pml_mask);
         const int o = (i + H) + ld * (j + H);
-        const float nv = P.vx[o] - P.cx[i + ld * j] * (d1 + d2);
-        P.vx[o] = nv;
+        float nv = P.vx[o] - P.cx[i + ld * j] * (d1 + d2);
         if ((P.rigid_mask & GPI_FACE(2, 0)) && j == 0) P.vx[(i + H)] = -nv;
-        if ((P.rigid_mask & GPI_FACE(2, 1)) && j == P.nx - H - 1)
-            P.vx[(i + H) + ld * (P.nx + H - 1)] = -nv;
+        if ((P.rigid_mask & GPI_FACE(2, 1)) && j == P.nx - H - 1) P.vx[(i + H) + ld * (P.nx + H - 1)] = -nv;
+        if (X.ninj) nv = inject_velocity(X, INJ_VX, b, o, nv, P.rho, P.dt);
+        P.vx[o] = nv;
     }
     // vz[inn] -= dtinvavzirho*(dtauxzdx + dtauzzdz) on (nz-h) x (nx-2h)
     if (i < P.nz - H && j < P.nx - 2 * H) {
@@ -194,20 +190,26 @@ __global__ __launch_bounds__(256) void vel2d_elastic(VelParams2D P)
         float d2 = dz_at<O>(P.s1, i + ld * (j + H)) * P.dzI;        // D_zi(tauzz)
         d2 = cpml_z(d2, P.qz0, i, j, P.npml, P.pml_mask);
         const int o = (i + H) + ld * (j + H);
-        const float nv = P.vz[o] - P.cz[i + ld * j] * (d1 + d2);
-        P.vz[o] = nv;
+        float nv = P.vz[o] - P.cz[i + ld * j] * (d1 + d2);
         if ((P.rigid_mask & GPI_FACE(0, 0)) && i == 0) P.vz[ld * (j + H)] = -nv;
-        if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1)
-            P.vz[(P.nz + H - 1) + ld * (j + H)] = -nv;
+        if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1) P.vz[(P.nz + H - 1) + ld * (j + H)] = -nv;
+        if (X.ninj) nv = inject_velocity(X, INJ_VZ, b, o, nv, P.rho, P.dt);
+        P.vz[o] = nv;
     }
 }
 
 template <int O>
-__global__ __launch_bounds__(256) void str2d_elastic(StrParams2D P)
+__global__ __launch_bounds__(256) void str2d_elastic(StrParams2D P, PassExtras X)
 {
     constexpr int H = O - 1;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int b = blockIdx.x;
+    const int nbm = P.gx * P.gy;
+    if (b >= nbm) {
+        gather_block(X, b - nbm);
+        return;
+    }
+    const int i = (b % P.gx) * 64 + threadIdx.x;
+    const int j = (b / P.gx) * 4 + threadIdx.y;
     const int ld = P.ld;
     const bool fs = (P.free_mask & GPI_FACE(0, 0)) != 0;
     if (i < P.nz && j < P.nx) {
@@ -217,46 +219,73 @@ __global__ __launch_bounds__(256) void str2d_elastic(StrParams2D P)
         float dvz = dz_at<O>(P.vz, o) * P.dzI;       // D_za(vz)
         dvz = cpml_z(dvz, P.qz0, i, j, P.npml, P.pml_mask);
         const float m = P.c0[o], l = P.c1[o];
-        P.s0[o] = (P.s0[o] - (m * dvx)) - (l * dvz);
-        const float tzz = (P.s1[o] - (m * dvz)) - (l * dvx);
-        // free surface at zmin: tauzz[1,:] = -tauzz[2,:] (advance_elastic.jl:227-230)
-        if (!(fs && i == 0)) P.s1[o] = tzz;
+        float txx = (P.s0[o] - (m * dvx)) - (l * dvz);
+        float tzz = (P.s1[o] - (m * dvz)) - (l * dvx);
+        // free surface at zmin, before the sources: tauzz[1,:] = -tauzz[2,:]
         if (fs && i == 1) P.s1[ld * j] = -tzz;
+        // tauxx += pv*dtM and tauzz += pv*dtM per source field (source.jl:79-97)
+        for (int q = 0; q < X.ninj; ++q) {
+            float acc;
+            int t;
+            if (X.inj[q].kind == INJ_NORMAL2 && injection_sum(X.inj[q], b, o, X.it, acc, t)) {
+                txx = txx + (acc * m);
+                tzz = tzz + (acc * m);
+            }
+        }
+        P.s0[o] = txx;
+        if (!(fs && i == 0)) P.s1[o] = tzz;
     }
     // tauxz -= dtavmu*(dvxdz + dvzdx) on (nz-h) x (nx-h)
     if (i < P.nz - H && j < P.nx - H) {
         const int o = i + ld * j;
-        float a = dz_at<O>(P.vx, i + ld * (j + H)) * P.dzI;   // D_zi(vx)
+        float a = dz_at<O>(P.vx, i + ld * (j + H)) * P.dzI;      // D_zi(vx)
         a = cpml_z(a, P.qz1, i, j, P.npml, P.pml_mask);
-        float b = dx_at<O>(P.vz, (i + H) + ld * j, ld) * P.dxI;  // D_xi(vz)
-        b = cpml_x(b, P.qx1, i, j, P.npml, P.pml_mask);
-        const float t = P.s2[o] - P.c2[o] * (a + b);
-        P.s2[o] = (fs && i == 0) ? 0.0f : t;   // tauxz[1,:] = 0
+        float c = dx_at<O>(P.vz, (i + H) + ld * j, ld) * P.dxI;  // D_xi(vz)
+        c = cpml_x(c, P.qx1, i, j, P.npml, P.pml_mask);
+        const float t = P.s2[o] - P.c2[o] * (a + c);
+        P.s2[o] = (fs && i == 0) ? 0.0f : t;                     // tauxz[1,:] = 0
     }
 }
 
 template <int O>
-void launch_vel(const Model& m, hipStream_t s, const VelParams2D& P)
+void launch_vel(const Model& m, hipStream_t s, VelParams2D P, const PassExtras& X)
 {
     constexpr int H = O - 1;
-    dim3 blk(64, 4);
-    dim3 grd((P.nz - H + 63) / 64, (P.nx - H + 3) / 4);
-    if (m.g.physics == 0) hipLaunchKernelGGL(vel2d_acoustic<O>, grd, blk, 0, s, P);
-    else hipLaunchKernelGGL(vel2d_elastic<O>, grd, blk, 0, s, P);
+    P.gx = (P.nz - H + 63) / 64;
+    P.gy = (P.nx - H + 3) / 4;
+    dim3 grd(P.gx * P.gy + X.gather_blocks), blk(64, 4);
+    if (m.g.physics == 0) hipLaunchKernelGGL(vel2d_acoustic<O>, grd, blk, 0, s, P, X);
+    else hipLaunchKernelGGL(vel2d_elastic<O>, grd, blk, 0, s, P, X);
 }
 
 template <int O>
-void launch_str(const Model& m, hipStream_t s, const StrParams2D& P)
+void launch_str(const Model& m, hipStream_t s, StrParams2D P, const PassExtras& X)
 {
-    dim3 blk(64, 4);
-    dim3 grd((P.nz + 63) / 64, (P.nx + 3) / 4);
-    if (m.g.physics == 0) hipLaunchKernelGGL(str2d_acoustic<O>, grd, blk, 0, s, P);
-    else hipLaunchKernelGGL(str2d_elastic<O>, grd, blk, 0, s, P);
+    P.gx = (P.nz + 63) / 64;
+    P.gy = (P.nx + 3) / 4;
+    dim3 grd(P.gx * P.gy + X.gather_blocks), blk(64, 4);
+    if (m.g.physics == 0) hipLaunchKernelGGL(str2d_acoustic<O>, grd, blk, 0, s, P, X);
+    else hipLaunchKernelGGL(str2d_elastic<O>, grd, blk, 0, s, P, X);
 }
 
 }  // namespace
 
-void velocity_pass_2d(const Model& m, hipStream_t s)
+int velocity_blocks_2d(const Model& m, int* gx, int* gy)
+{
+    const int H = m.h;
+    *gx = (int)((m.g.n[0] - H + 63) / 64);
+    *gy = (int)((m.g.n[2] - H + 3) / 4);
+    return *gx * *gy;
+}
+
+int stress_blocks_2d(const Model& m, int* gx, int* gy)
+{
+    *gx = (int)((m.g.n[0] + 63) / 64);
+    *gy = (int)((m.g.n[2] + 3) / 4);
+    return *gx * *gy;
+}
+
+void velocity_pass_2d(const Model& m, hipStream_t s, const PassExtras& X)
 {
     VelParams2D P{};
     const bool ac = m.g.physics == 0;
@@ -267,6 +296,7 @@ void velocity_pass_2d(const Model& m, hipStream_t s)
     P.vz = m.field[GPI_VZ];
     P.cx = m.coef[GPI_C_DTINVAVXIRHO];
     P.cz = m.coef[GPI_C_DTINVAVZIRHO];
+    P.rho = m.coef[GPI_C_RHO];
     P.qx0 = m.pml[ac ? GPI_DPDX : GPI_DTAUXXDX];
     P.qx1 = ac ? Pml{} : m.pml[GPI_DTAUXZDX];
     P.qz0 = m.pml[ac ? GPI_DPDZ : GPI_DTAUZZDZ];
@@ -279,15 +309,16 @@ void velocity_pass_2d(const Model& m, hipStream_t s)
     P.rigid_mask = m.g.rigid_mask;
     P.dxI = m.g.dsI[2];
     P.dzI = m.g.dsI[0];
+    P.dt = m.g.dt;
     switch (m.g.order) {
-    case 2: launch_vel<2>(m, s, P); break;
-    case 4: launch_vel<4>(m, s, P); break;
-    case 6: launch_vel<6>(m, s, P); break;
-    default: launch_vel<8>(m, s, P); break;
+    case 2: launch_vel<2>(m, s, P, X); break;
+    case 4: launch_vel<4>(m, s, P, X); break;
+    case 6: launch_vel<6>(m, s, P, X); break;
+    default: launch_vel<8>(m, s, P, X); break;
     }
 }
 
-void stress_pass_2d(const Model& m, hipStream_t s)
+void stress_pass_2d(const Model& m, hipStream_t s, const PassExtras& X)
 {
     StrParams2D P{};
     const bool ac = m.g.physics == 0;
@@ -312,10 +343,10 @@ void stress_pass_2d(const Model& m, hipStream_t s)
     P.dxI = m.g.dsI[2];
     P.dzI = m.g.dsI[0];
     switch (m.g.order) {
-    case 2: launch_str<2>(m, s, P); break;
-    case 4: launch_str<4>(m, s, P); break;
-    case 6: launch_str<6>(m, s, P); break;
-    default: launch_str<8>(m, s, P); break;
+    case 2: launch_str<2>(m, s, P, X); break;
+    case 4: launch_str<4>(m, s, P, X); break;
+    case 6: launch_str<6>(m, s, P, X); break;
+    default: launch_str<8>(m, s, P, X); break;
     }
 }
 

@@ -6,7 +6,9 @@
 #include <map>
 #include <new>
 
+#include "fdtd2d.h"
 #include "gpi_internal.h"
+#include "pass_extras.h"
 
 using namespace gpi;
 
@@ -477,8 +479,13 @@ int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
     Scratch S(c);
     const int nnz_expect = g.ndim == 2 ? 4 : 8;
 
-    // ---- sources: scatter form with unique targets --------------------------------------
+    // ---- sources: unique targets, grouped by the workgroup of the pass that owns them ----
+    int vgx, vgy, sgx, sgy;
+    const int vnb = velocity_blocks_2d(m, &vgx, &vgy);
+    const int snb = stress_blocks_2d(m, &sgx, &sgy);
     std::vector<DevPoints> src(shot->nsrc);
+    std::vector<int> kinds(shot->nsrc);
+    bool vel_fused = true, str_fused = true;
     for (int q = 0; q < shot->nsrc; ++q) {
         const gpi_points& P = shot->src[q];
         const int f = P.field;
@@ -489,6 +496,8 @@ int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
         if (!vel && !str) return fail(c, GPI_EINVAL, "shear stresses cannot carry sources");
         if (P.nnz != nnz_expect || P.npts < 0 || (P.npts > 0 && (!P.idx || !P.w || !shot->wavelets || !shot->wavelets[q])))
             return fail(c, GPI_EINVAL, "bad source point set");
+        kinds[q] = f == GPI_VX ? INJ_VX : f == GPI_VY ? INJ_VY : f == GPI_VZ ? INJ_VZ
+                 : g.physics == 0 ? INJ_P : (g.ndim == 3 ? INJ_NORMAL3 : INJ_NORMAL2);
         int64_t n3[3];
         shape_of(g.ndim, g.order, g.physics, g.n, f, n3);
         const int64_t N = n3[0] * n3[1] * n3[2];
@@ -501,6 +510,38 @@ int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
                 if (lin < 0 || lin >= N) return fail(c, GPI_EINVAL, "source index out of range");
                 tg[lin].push_back({s, P.w[(size_t)s * P.nnz + k]});
             }
+        struct Tgt {
+            int64_t lin;
+            int blk;
+            const std::vector<std::pair<int, float>>* con;
+        };
+        std::vector<Tgt> tl;
+        for (auto& kv : tg) {
+            const int64_t lin = kv.first;
+            const int64_t iz = lin % n3[0], r = lin / n3[0], ix = r / n3[1];
+            int blk;
+            if (vel) {
+                // muladd_with_density_v*! touches @inn(v) only (source.jl:166-177)
+                if (!(iz >= h && iz < n3[0] - h && ix >= h && ix < n3[2] - h)) continue;
+                blk = (int)((iz - h) / 64 + vgx * ((ix - h) / 4));
+            } else {
+                blk = (int)(iz / 64 + sgx * (ix / 4));
+                // a source on the free-surface row is applied after the surface condition
+                // by a separate kernel (source.jl:61-119 runs after update_stress!)
+                if ((g.free_mask & GPI_FACE(0, 0)) && iz == 0) str_fused = false;
+            }
+            tl.push_back({lin, blk, &kv.second});
+        }
+        std::stable_sort(tl.begin(), tl.end(), [](const Tgt& x, const Tgt& y) { return x.blk < y.blk; });
+        const int nb = vel ? vnb : snb;
+        std::vector<int32_t> blk(nb + 1, 0);
+        for (const Tgt& t : tl) blk[t.blk + 1]++;
+        int maxper = 0;
+        for (int k = 0; k < nb; ++k) {
+            maxper = std::max(maxper, blk[k + 1]);
+            blk[k + 1] += blk[k];
+        }
+        if (maxper > 32) (vel ? vel_fused : str_fused) = false;   // keep owner scans short
         DevPoints& D = src[q];
         D.field = f;
         D.npts = P.npts;
@@ -508,26 +549,20 @@ int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
         std::vector<int64_t> toff, roff;
         std::vector<int32_t> tptr{0}, tsrc;
         std::vector<float> tw;
-        for (auto& kv : tg) {
-            const int64_t lin = kv.first;
-            int64_t iz = lin % n3[0], r = lin / n3[0], iy = r % n3[1], ix = r / n3[1];
+        for (const Tgt& t : tl) {
             if (vel) {
-                // muladd_with_density_v*! touches @inn(v) only (source.jl:166-177)
-                const bool inz = iz >= h && iz < n3[0] - h;
-                const bool inx = ix >= h && ix < n3[2] - h;
-                const bool iny = g.ndim == 2 || (iy >= h && iy < n3[1] - h);
-                if (!(inz && inx && iny)) continue;
                 // @av_{x,y,z}i(rho) at the interior point: rho indices of the two samples
-                int64_t b[3] = {iz, iy, ix};
+                const int64_t iz = t.lin % n3[0], r = t.lin / n3[0], iy = r % n3[1], ix = r / n3[1];
+                int64_t b0[3] = {iz, iy, ix};
                 const int ax = f == GPI_VZ ? 0 : f == GPI_VY ? 1 : 2;
-                b[ax] -= h;
-                int64_t b2[3] = {b[0], b[1], b[2]};
-                b2[ax] += 1;
-                roff.push_back(b[0] + m.pitch0 * (b[1] + m.pitch1 * b[2]));
-                roff.push_back(b2[0] + m.pitch0 * (b2[1] + m.pitch1 * b2[2]));
+                b0[ax] -= h;
+                int64_t b1[3] = {b0[0], b0[1], b0[2]};
+                b1[ax] += 1;
+                roff.push_back(b0[0] + m.pitch0 * (b0[1] + m.pitch1 * b0[2]));
+                roff.push_back(b1[0] + m.pitch0 * (b1[1] + m.pitch1 * b1[2]));
             }
-            toff.push_back(padded(m, n3, lin));
-            for (auto& pr : kv.second) {
+            toff.push_back(padded(m, n3, t.lin));
+            for (auto& pr : *t.con) {
                 tsrc.push_back(pr.first);
                 tw.push_back(pr.second);
             }
@@ -537,7 +572,7 @@ int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
         toff.insert(toff.end(), roff.begin(), roff.end());
         int rc;
         if ((rc = S.put(&D.toff, toff)) || (rc = S.put(&D.tptr, tptr)) || (rc = S.put(&D.tsrc, tsrc)) ||
-            (rc = S.put(&D.tw, tw)))
+            (rc = S.put(&D.tw, tw)) || (rc = S.put(&D.blk, blk)))
             return rc;
         // wavelets: host nt x npts time fastest -> device point fastest per step
         std::vector<float> wv((size_t)nt * std::max(P.npts, 1), 0.0f);
@@ -548,7 +583,8 @@ int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
 
     // ---- receivers: gather form ---------------------------------------------------------
     std::vector<DevPoints> rec(shot->nrec);
-    std::vector<int> rec_active;
+    PassExtras XV{}, XS{}, X0{};
+    int vmax = 0, smax = 0;
     for (int q = 0; q < shot->nrec; ++q) {
         const gpi_points& P = shot->rec[q];
         const int f = P.field;
@@ -573,10 +609,55 @@ int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
         if ((rc = S.put(&D.idx, off)) || (rc = S.put(&D.w, w))) return rc;
         if ((rc = S.alloc(&D.rec, (size_t)nt * std::max(P.npts, 1)))) return rc;
         HIPTRY(c, hipMemsetAsync(D.rec, 0, sizeof(float) * (size_t)nt * std::max(P.npts, 1), c->stream));
-        if (recordable(g.physics, f) && P.npts > 0) rec_active.push_back(q);
+        if (!recordable(g.physics, f) || P.npts == 0) continue;
+        // p is sampled by the velocity pass (before it), v by the stress pass (after v's sources)
+        const bool vfield = f == GPI_VX || f == GPI_VY || f == GPI_VZ;
+        GatherSet& G = vfield ? XS.g : XV.g;
+        if (G.nset == 4) return fail(c, GPI_EINVAL, "too many receiver fields");
+        const int k = G.nset++;
+        G.nnz = P.nnz;
+        G.npts[k] = P.npts;
+        G.field[k] = m.field[f];
+        G.idx[k] = D.idx;
+        G.w[k] = D.w;
+        G.rec[k] = D.rec;
+        int& mx = vfield ? smax : vmax;
+        mx = std::max(mx, P.npts);
     }
-    std::vector<DevPoints> recs_on;
-    for (int q : rec_active) recs_on.push_back(rec[q]);
+    XV.gather_blocks = XV.g.nset * ((vmax + 255) / 256);
+    XS.gather_blocks = XS.g.nset * ((smax + 255) / 256);
+
+    // ---- injection sets riding along with the passes ------------------------------------
+    std::vector<int> vfall, sfall;
+    for (int q = 0; q < shot->nsrc; ++q) {
+        const DevPoints& D = src[q];
+        if (D.ntarget == 0) continue;
+        const bool vel = kinds[q] >= INJ_VX;
+        PassExtras& X = vel ? XV : XS;
+        bool& fused = vel ? vel_fused : str_fused;
+        if (fused && X.ninj == MAX_INJ) fused = false;
+        if (fused) {
+            InjSet& I = X.inj[X.ninj++];
+            I.kind = kinds[q];
+            I.npts = D.npts;
+            I.blk = D.blk;
+            I.toff = D.toff;
+            I.tptr = D.tptr;
+            I.tsrc = D.tsrc;
+            I.tw = D.tw;
+            I.wav = D.wav;
+            I.roff = D.toff + D.ntarget;
+        }
+    }
+    // a pass either applies all of its source sets or none (keeps the reference's order)
+    if (!vel_fused) XV.ninj = 0;
+    if (!str_fused) XS.ninj = 0;
+    for (int q = 0; q < shot->nsrc; ++q) {
+        if (src[q].ntarget == 0) continue;
+        const bool vel = kinds[q] >= INJ_VX;
+        if (vel && !vel_fused) vfall.push_back(q);
+        if (!vel && !str_fused) sfall.push_back(q);
+    }
 
     // ---- reset wavefields and CPML memory (reset_w2!, types.jl:101-113) ----------------
     for (int f = GPI_P; f <= GPI_TAUYZ; ++f)
@@ -592,13 +673,8 @@ int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
     }
 
     // ---- time loop (propagate.jl:170-258) ------------------------------------------------
-    std::vector<int> vsrc, ssrc;
-    for (int q = 0; q < shot->nsrc; ++q) {
-        const int f = src[q].field;
-        if (f == GPI_VX || f == GPI_VY || f == GPI_VZ) vsrc.push_back(q);
-        else ssrc.push_back(q);
-    }
-    // sampled steps for per-pass timing: evenly spaced over the run
+    // per step: velocity pass (records p at :177, sources of :202 ride along), then stress
+    // pass (records v at :208, sources of :223 ride along)
     const int64_t nsamp = std::min<int64_t>((int64_t)c->sample_ev.size() / 4, nt);
     int64_t next_sample = nsamp ? 0 : -1, isamp = 0;
     int64_t launches = 0;
@@ -606,29 +682,23 @@ int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
     for (int64_t it = 0; it < nt; ++it) {
         const bool samp = it == next_sample;
         hipEvent_t* E = samp ? &c->sample_ev[4 * isamp] : nullptr;
-        if (!recs_on.empty()) {
-            gather(m, recs_on.data(), (int)recs_on.size(), it, true, c->stream);   // :177
-            ++launches;
-        }
+        XV.it = it;
+        XS.it = it;
         if (samp) HIPTRY(c, hipEventRecord(E[0], c->stream));
-        velocity_pass_2d(m, c->stream);                                              // :192-197
+        velocity_pass_2d(m, c->stream, XV);                                          // :177-202
         if (samp) HIPTRY(c, hipEventRecord(E[1], c->stream));
-        for (int q : vsrc) inject(m, src[q], it, c->stream);                         // :202
-        if (!recs_on.empty()) {
-            gather(m, recs_on.data(), (int)recs_on.size(), it, false, c->stream);  // :208
-            ++launches;
-        }
+        for (int q : vfall) inject(m, src[q], it, c->stream);
         if (samp) HIPTRY(c, hipEventRecord(E[2], c->stream));
-        stress_pass_2d(m, c->stream);                                                // :213-218
+        stress_pass_2d(m, c->stream, XS);                                            // :208-223
         if (samp) HIPTRY(c, hipEventRecord(E[3], c->stream));
-        for (int q : ssrc) inject(m, src[q], it, c->stream);                         // :223
-        launches += 2 + (int64_t)vsrc.size() + (int64_t)ssrc.size();
+        for (int q : sfall) inject(m, src[q], it, c->stream);
+        launches += 2 + (int64_t)vfall.size() + (int64_t)sfall.size();
         if (samp) {
             ++isamp;
             next_sample = isamp < nsamp ? (isamp * nt) / nsamp : -1;
         }
     }
-    velocity_pass_2d(m, c->stream);                                                  // :254-255
+    velocity_pass_2d(m, c->stream, X0);                                              // :254-255
     ++launches;
     HIPTRY(c, hipEventRecord(c->ev[1], c->stream));
     HIPTRY(c, hipGetLastError());

@@ -41,6 +41,8 @@ struct DevPoints {
     int64_t* idx = nullptr;     // gather: npts*nnz padded-layout offsets
     float* w = nullptr;         // gather: npts*nnz weights
     int32_t ntarget = 0;        // scatter: unique target cells (interior only for velocity)
+    int32_t* blk = nullptr;     // scatter: per-workgroup target ranges of the owning pass
+    bool fused = false;         // scatter: applied inside the owning pass
     int64_t* toff = nullptr;    // scatter: padded offsets of targets
     int32_t* tptr = nullptr;    // scatter: ntarget+1 CSR row pointers
     int32_t* tsrc = nullptr;    // scatter: source column per contribution
@@ -73,16 +75,8 @@ struct Ctx {
     gpi_timing timing{};
 };
 
-// kernel launchers (fdtd2d.hip)
-struct Launch2D {
-    const Model* m;
-    hipStream_t s;
-};
-void velocity_pass_2d(const Model& m, hipStream_t s);
-void stress_pass_2d(const Model& m, hipStream_t s);
-
-// sparse kernels (sparse.hip)
+// stand-alone sparse kernels (sparse.hip): injection after a pass, for source sets that
+// cannot ride along with the pass (targets on a free-surface row or dense target lists)
 void inject(const Model& m, const DevPoints& pts, int64_t it, hipStream_t s);
-void gather(const Model& m, const DevPoints* pts, int n, int64_t it, bool stress, hipStream_t s);
 
 }  // namespace gpi

@@ -1,0 +1,103 @@
+// Work that rides along with a stencil pass: source injection by the thread that owns the
+// updated cell, and receiver sampling by extra workgroups appended to the launch.
+//
+// Injection: targets of one source field are sorted by the workgroup that owns them;
+// blk[b]..blk[b+1] lists the targets of workgroup b, so a workgroup without targets pays
+// two cached loads.  The owner sums the contributions in ascending source order (the CSC
+// spmv order of src/fdtd/source.jl:71,89,109,136,152) and applies the reference's update
+// expression to the value it has just computed, so the result equals the separate
+// injection pass of the reference bit for bit.
+//
+// Sampling: a pass reads but never writes the fields it samples (p before the velocity
+// update, v before the stress update), so extra workgroups can gather receivers
+// concurrently with the stencil workgroups (src/fdtd/receiver.jl:3-14).
+#pragma once
+#include "gpi_internal.h"
+
+namespace gpi {
+
+enum InjKind { INJ_P = 0, INJ_NORMAL2 = 1, INJ_NORMAL3 = 2, INJ_VX = 3, INJ_VY = 4, INJ_VZ = 5 };
+
+struct InjSet {
+    int kind;
+    int npts;              // sources of the field (wavelet row length)
+    const int32_t* blk;    // nblocks+1 target ranges per workgroup
+    const int64_t* toff;   // padded offsets of the targets, grouped by workgroup
+    const int32_t* tptr;   // contribution ranges per target
+    const int32_t* tsrc;
+    const float* tw;
+    const float* wav;      // nt x npts, point fastest
+    const int64_t* roff;   // velocity sources: two density offsets per target
+};
+
+struct GatherSet {
+    int nset;
+    int nnz;
+    int npts[4];
+    const float* field[4];
+    const int64_t* idx[4];
+    const float* w[4];
+    float* rec[4];
+};
+
+constexpr int MAX_INJ = 4;
+
+struct PassExtras {
+    int64_t it;
+    int ninj;
+    InjSet inj[MAX_INJ];
+    int gather_blocks;     // extra workgroups appended after the stencil workgroups
+    GatherSet g;
+};
+
+// Sum of the contributions to target `off` of workgroup `b`; returns false if none.
+__device__ __forceinline__ bool injection_sum(const InjSet& S, int b, int64_t off, int64_t it, float& acc,
+                                              int& tslot)
+{
+    const int s = S.blk[b], e = S.blk[b + 1];
+    for (int t = s; t < e; ++t) {
+        if (S.toff[t] == off) {
+            const float* w = S.wav + it * (int64_t)S.npts;
+            float a = 0.0f;
+            for (int c = S.tptr[t]; c < S.tptr[t + 1]; ++c) a += S.tw[c] * w[S.tsrc[c]];
+            acc = a;
+            tslot = t;
+            return true;
+        }
+    }
+    return false;
+}
+
+// v[inn] = v[inn] + (pv / @av(rho)) * dt  (muladd_with_density_v*!, source.jl:166-177)
+__device__ __forceinline__ float inject_velocity(const PassExtras& X, int kind, int b, int64_t off, float v,
+                                                 const float* rho, float dt)
+{
+    for (int q = 0; q < X.ninj; ++q) {
+        const InjSet& S = X.inj[q];
+        if (S.kind != kind) continue;
+        float acc;
+        int t;
+        if (injection_sum(S, b, off, X.it, acc, t)) {
+            const float av = (rho[S.roff[2 * t]] + rho[S.roff[2 * t + 1]]) * 0.5f;
+            v = v + ((acc / av) * dt);
+        }
+    }
+    return v;
+}
+
+// receiver sampling by workgroup gb of the extra range (one lane per receiver)
+__device__ __forceinline__ void gather_block(const PassExtras& X, int gb)
+{
+    const int per = (X.gather_blocks + X.g.nset - 1) / X.g.nset;
+    const int s = gb / per;
+    const int r = (gb - s * per) * (int)(blockDim.x * blockDim.y) + (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    if (s >= X.g.nset || r >= X.g.npts[s]) return;
+    const int64_t* ix = X.g.idx[s] + (int64_t)r * X.g.nnz;
+    const float* ww = X.g.w[s] + (int64_t)r * X.g.nnz;
+    const float* F = X.g.field[s];
+    float tmp = 0.0f;
+    for (int k = 0; k < X.g.nnz; ++k) tmp += ww[k] * F[ix[k]];
+    X.g.rec[s][X.it * (int64_t)X.g.npts[s] + r] = tmp;
+}
+
+}  // namespace gpi

@@ -1,4 +1,4 @@
-// Source injection and receiver sampling kernels.
+// Stand-alone source injection kernel.
 //
 // Injection replaces the reference's two full-grid passes per source field and step
 // (CSC spmv into a zeroed grid buffer, then a whole-grid muladd: src/fdtd/source.jl:61-177)
@@ -6,9 +6,8 @@
 // source order, which is the accumulation order of SparseArrays' CSC mul!, then applies the
 // reference's update expression, so values are identical.
 //
-// Sampling (src/fdtd/receiver.jl:3-14, transpose(W)*field) runs one lane per receiver and
-// sums its 4 (2-D) or 8 (3-D) weighted samples in ascending index order, like the
-// column-wise dot product of SparseArrays.
+// The stencil passes normally apply injection themselves (pass_extras.h); this stand-alone
+// kernel serves the source sets that cannot ride along with a pass.
 #include "gpi_internal.h"
 
 namespace gpi {
@@ -62,30 +61,6 @@ __global__ __launch_bounds__(256) void inject_kernel(InjectParams P)
     }
 }
 
-struct GatherParams {
-    int nset;
-    int nnz;
-    int64_t it;
-    int npts[4];
-    const float* field[4];
-    const int64_t* idx[4];
-    const float* w[4];
-    float* rec[4];
-};
-
-__global__ __launch_bounds__(256) void gather_kernel(GatherParams P)
-{
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = blockIdx.y;
-    if (s >= P.nset || r >= P.npts[s]) return;
-    const int64_t* ix = P.idx[s] + (int64_t)r * P.nnz;
-    const float* ww = P.w[s] + (int64_t)r * P.nnz;
-    const float* F = P.field[s];
-    float tmp = 0.0f;
-    for (int k = 0; k < P.nnz; ++k) tmp += ww[k] * F[ix[k]];
-    P.rec[s][P.it * (int64_t)P.npts[s] + r] = tmp;
-}
-
 }  // namespace
 
 void inject(const Model& m, const DevPoints& pts, int64_t it, hipStream_t s)
@@ -119,34 +94,6 @@ void inject(const Model& m, const DevPoints& pts, int64_t it, hipStream_t s)
         P.coef = m.coef[GPI_C_DTM];
     }
     hipLaunchKernelGGL(inject_kernel, dim3((pts.ntarget + 255) / 256), dim3(256), 0, s, P);
-}
-
-void gather(const Model& m, const DevPoints* pts, int n, int64_t it, bool stress, hipStream_t s)
-{
-    GatherParams P{};
-    int maxn = 0;
-    for (int q = 0; q < n; ++q) {
-        const int f = pts[q].field;
-        const bool is_stress = !(f == GPI_VX || f == GPI_VY || f == GPI_VZ);
-        if (is_stress != stress) continue;
-        if (P.nset == 4) {
-            P.it = it;
-            hipLaunchKernelGGL(gather_kernel, dim3((maxn + 255) / 256, P.nset), dim3(256), 0, s, P);
-            P.nset = 0;
-            maxn = 0;
-        }
-        const int k = P.nset++;
-        P.nnz = pts[q].nnz;
-        P.npts[k] = pts[q].npts;
-        P.field[k] = m.field[f];
-        P.idx[k] = pts[q].idx;
-        P.w[k] = pts[q].w;
-        P.rec[k] = pts[q].rec;
-        if (pts[q].npts > maxn) maxn = pts[q].npts;
-    }
-    if (P.nset == 0 || maxn == 0) return;
-    P.it = it;
-    hipLaunchKernelGGL(gather_kernel, dim3((maxn + 255) / 256, P.nset), dim3(256), 0, s, P);
 }
 
 }  // namespace gpi

@@ -73,6 +73,8 @@ def elastic_layers_2d(n: int = 2001, ds: float = 2.5, nt: int = 4000, npml: int 
     ageom = [AGeomss({"z": [10 * ds], "x": [L / 2]},
                      {"z": np.zeros(nrec), "x": np.linspace(0.0, L, nrec)})]
     src = Srcs(1, tgrid, ["tauxx"])
-    src.update(["tauxx"], ricker(freq, tgrid))
+    # a run shorter than the wavelet keeps its first nt samples
+    nw = max(nt, int(math.ceil(3.2 / freq / dt)))
+    src.update(["tauxx"], ricker(freq, Grid(0.0, dt, nw))[:nt])
     kw = dict(rfields=["vx", "vz"], npml=npml)
     return medium, ageom, [src], tgrid, kw
