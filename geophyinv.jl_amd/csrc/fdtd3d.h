@@ -1,0 +1,53 @@
+// Parameters and launchers of the 3-D stencil passes.
+#pragma once
+#include "gpi_internal.h"
+#include "pass_extras.h"
+
+namespace gpi {
+
+struct VelParams3D {
+    // acoustic: s[0] = p; elastic: s = tauxx, tauyy, tauzz, tauxy, tauxz, tauyz
+    const float* s[6];
+    float* vx;
+    float* vy;
+    float* vz;
+    const float* cx;   // dtinvavxirho
+    const float* cy;   // dtinvavyirho
+    const float* cz;   // dtinvavzirho
+    const float* rho;
+    // CPML of the derivatives; acoustic: x[0] dpdx, y[0] dpdy, z[0] dpdz
+    // elastic: vx uses x[0] dtauxxdx, y[0] dtauxydy, z[0] dtauxzdz
+    //          vy uses x[1] dtauxydx, y[1] dtauyydy, z[1] dtauyzdz
+    //          vz uses x[2] dtauxzdx, y[2] dtauyzdy, z[2] dtauzzdz
+    Pml qx[3], qy[3], qz[3];
+    int nz, ny, nx, npml;
+    int64_t s1, s2;    // y and x strides of the pitched layout
+    int gx, gy;        // workgroups along z and y; the grid's third factor runs over x
+    uint32_t pml_mask, rigid_mask;
+    float dxI, dyI, dzI, dt;
+};
+
+struct StrParams3D {
+    const float* vx;
+    const float* vy;
+    const float* vz;
+    // acoustic: s[0] = p; elastic: tauxx, tauyy, tauzz, tauxy, tauxz, tauyz
+    float* s[6];
+    // acoustic: c[0] = dtK; elastic: dtM, dtlambda, dtavxyimu, dtavxzimu, dtavyzimu
+    const float* c[5];
+    // acoustic: x[0] dvxdx, y[0] dvydy, z[0] dvzdz
+    // elastic:  x: dvxdx, dvydx, dvzdx; y: dvydy, dvxdy, dvzdy; z: dvzdz, dvxdz, dvydz
+    Pml qx[3], qy[3], qz[3];
+    int nz, ny, nx, npml;
+    int64_t s1, s2;
+    int gx, gy;
+    uint32_t pml_mask, free_mask;
+    float dxI, dyI, dzI;
+};
+
+int velocity_blocks_3d(const Model& m, int* gx, int* gy);
+int stress_blocks_3d(const Model& m, int* gx, int* gy);
+void velocity_pass_3d(const Model& m, hipStream_t s, const PassExtras& X);
+void stress_pass_3d(const Model& m, hipStream_t s, const PassExtras& X);
+
+}  // namespace gpi

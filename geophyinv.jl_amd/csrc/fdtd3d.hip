@@ -1,0 +1,442 @@
+// 3-D staggered-grid velocity-stress kernels for gfx950.
+//
+// Same structure as the 2-D passes: the velocity pass fuses the stress derivatives, their
+// CPML strips, compute_v!, the rigid faces and the velocity sources
+// (src/fdtd/advance_acou.jl:98-194, src/fdtd/advance_elastic.jl:233-394); the stress pass
+// fuses the velocity derivatives, CPML, compute_p!/compute_stressii!/compute_stressij!, the
+// free surface and the stress sources (src/fdtd/advance_acou.jl:196-250,
+// src/fdtd/advance_elastic.jl:395-531).  Workgroups are 64 (z) x 4 (y) threads on one x
+// plane; workgroup b covers z-rows 64*(b % gx).., y-rows 4*((b / gx) % gy).., plane
+// b / (gx*gy).  Extra workgroups after the stencil range sample receivers.
+#include "fdtd3d.h"
+#include "pass_extras.h"
+#include "stencil.h"
+
+namespace gpi {
+
+namespace {
+
+__device__ __forceinline__ float cpml_update(float d, const Pml& q, int idx, int c)
+{
+    float* m = q.psi + idx;
+    const float mv = q.b[c] * *m + q.a[c] * d;
+    *m = mv;
+    return d * q.kI[c] + mv;
+}
+
+// CPML of a derivative along x at derivative-grid point (i, j, k); q.nd = its x length
+__device__ __forceinline__ float cpml3_x(float d, const Pml& q, int i, int j, int k, int npml, uint32_t mask)
+{
+    if ((mask & GPI_FACE(2, 0)) && k < npml) d = cpml_update(d, q, i + (int)q.ps1 * j + (int)q.ps2 * k, k);
+    if ((mask & GPI_FACE(2, 1)) && k >= q.nd - npml) {
+        const int c = k - (q.nd - npml) + npml;
+        d = cpml_update(d, q, i + (int)q.ps1 * j + (int)q.ps2 * c, c);
+    }
+    return d;
+}
+
+__device__ __forceinline__ float cpml3_y(float d, const Pml& q, int i, int j, int k, int npml, uint32_t mask)
+{
+    if ((mask & GPI_FACE(1, 0)) && j < npml) d = cpml_update(d, q, i + (int)q.ps1 * j + (int)q.ps2 * k, j);
+    if ((mask & GPI_FACE(1, 1)) && j >= q.nd - npml) {
+        const int r = j - (q.nd - npml) + npml;
+        d = cpml_update(d, q, i + (int)q.ps1 * r + (int)q.ps2 * k, r);
+    }
+    return d;
+}
+
+__device__ __forceinline__ float cpml3_z(float d, const Pml& q, int i, int j, int k, int npml, uint32_t mask)
+{
+    if ((mask & GPI_FACE(0, 0)) && i < npml) d = cpml_update(d, q, i + (int)q.ps1 * j + (int)q.ps2 * k, i);
+    if ((mask & GPI_FACE(0, 1)) && i >= q.nd - npml) {
+        const int r = i - (q.nd - npml) + npml;
+        d = cpml_update(d, q, r + (int)q.ps1 * j + (int)q.ps2 * k, r);
+    }
+    return d;
+}
+
+template <int O>
+__device__ __forceinline__ float dstride(const float* a, int off, int stride)
+{
+    float v[O];
+#pragma unroll
+    for (int m = 0; m < O; ++m) v[m] = a[off + m * stride];
+    return stencil<O>(v);
+}
+
+struct Idx3 {
+    int i, j, k;
+};
+
+__device__ __forceinline__ bool decode(int b, int gx, int gy, Idx3& p)
+{
+    const int bx = b % gx;
+    const int r = b / gx;
+    const int by = r % gy;
+    p.k = r / gy;
+    p.i = bx * 64 + (int)threadIdx.x;
+    p.j = by * 4 + (int)threadIdx.y;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------
+// acoustic
+// ---------------------------------------------------------------------------------------
+
+template <int O>
+__global__ __launch_bounds__(256) void vel3d_acoustic(VelParams3D P, PassExtras X)
+{
+    constexpr int H = O - 1;
+    const int b = blockIdx.x;
+    const int nxw = P.nx - H;
+    if (b >= P.gx * P.gy * nxw) {
+        gather_block(X, b - P.gx * P.gy * nxw);
+        return;
+    }
+    Idx3 t;
+    decode(b, P.gx, P.gy, t);
+    const int i = t.i, j = t.j, k = t.k;
+    const int s1 = (int)P.s1, s2 = (int)P.s2;
+    const float* p = P.s[0];
+    const int o = (i + H) + s1 * (j + H) + s2 * (k + H);   // @inn of every velocity
+    const int c = i + s1 * j + s2 * k;                     // derivative-grid point
+    // vx on (nz-2h, ny-2h, nx-h): dpdx = D_xi(p)
+    if (i < P.nz - 2 * H && j < P.ny - 2 * H && k < P.nx - H) {
+        float d = dstride<O>(p, (i + H) + s1 * (j + H) + s2 * k, s2) * P.dxI;
+        d = cpml3_x(d, P.qx[0], i, j, k, P.npml, P.pml_mask);
+        float nv = P.vx[o] + P.cx[c] * d;
+        if ((P.rigid_mask & GPI_FACE(2, 0)) && k == 0) P.vx[(i + H) + s1 * (j + H)] = -nv;
+        if ((P.rigid_mask & GPI_FACE(2, 1)) && k == P.nx - H - 1)
+            P.vx[(i + H) + s1 * (j + H) + s2 * (P.nx + H - 1)] = -nv;
+        if (X.ninj) nv = inject_velocity(X, INJ_VX, b, o, nv, P.rho, P.dt);
+        P.vx[o] = nv;
+    }
+    // vy on (nz-2h, ny-h, nx-2h): dpdy = D_yi(p)
+    if (i < P.nz - 2 * H && j < P.ny - H && k < P.nx - 2 * H) {
+        float d = dstride<O>(p, (i + H) + s1 * j + s2 * (k + H), s1) * P.dyI;
+        d = cpml3_y(d, P.qy[0], i, j, k, P.npml, P.pml_mask);
+        float nv = P.vy[o] + P.cy[c] * d;
+        if ((P.rigid_mask & GPI_FACE(1, 0)) && j == 0) P.vy[(i + H) + s2 * (k + H)] = -nv;
+        if ((P.rigid_mask & GPI_FACE(1, 1)) && j == P.ny - H - 1)
+            P.vy[(i + H) + s1 * (P.ny + H - 1) + s2 * (k + H)] = -nv;
+        if (X.ninj) nv = inject_velocity(X, INJ_VY, b, o, nv, P.rho, P.dt);
+        P.vy[o] = nv;
+    }
+    // vz on (nz-h, ny-2h, nx-2h): dpdz = D_zi(p)
+    if (i < P.nz - H && j < P.ny - 2 * H && k < P.nx - 2 * H) {
+        float d = dstride<O>(p, i + s1 * (j + H) + s2 * (k + H), 1) * P.dzI;
+        d = cpml3_z(d, P.qz[0], i, j, k, P.npml, P.pml_mask);
+        float nv = P.vz[o] + P.cz[c] * d;
+        if ((P.rigid_mask & GPI_FACE(0, 0)) && i == 0) P.vz[s1 * (j + H) + s2 * (k + H)] = -nv;
+        if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1)
+            P.vz[(P.nz + H - 1) + s1 * (j + H) + s2 * (k + H)] = -nv;
+        if (X.ninj) nv = inject_velocity(X, INJ_VZ, b, o, nv, P.rho, P.dt);
+        P.vz[o] = nv;
+    }
+}
+
+template <int O>
+__global__ __launch_bounds__(256) void str3d_acoustic(StrParams3D P, PassExtras X)
+{
+    const int b = blockIdx.x;
+    if (b >= P.gx * P.gy * P.nx) {
+        gather_block(X, b - P.gx * P.gy * P.nx);
+        return;
+    }
+    Idx3 t;
+    decode(b, P.gx, P.gy, t);
+    const int i = t.i, j = t.j, k = t.k;
+    if (i >= P.nz || j >= P.ny) return;
+    const int s1 = (int)P.s1, s2 = (int)P.s2;
+    const int o = i + s1 * j + s2 * k;
+    float dvx = dstride<O>(P.vx, o, s2) * P.dxI;     // D_xa(vx)
+    dvx = cpml3_x(dvx, P.qx[0], i, j, k, P.npml, P.pml_mask);
+    float dvy = dstride<O>(P.vy, o, s1) * P.dyI;     // D_ya(vy)
+    dvy = cpml3_y(dvy, P.qy[0], i, j, k, P.npml, P.pml_mask);
+    float dvz = dstride<O>(P.vz, o, 1) * P.dzI;      // D_za(vz)
+    dvz = cpml3_z(dvz, P.qz[0], i, j, k, P.npml, P.pml_mask);
+    const float c = P.c[0][o];
+    float pn = P.s[0][o] + ((dvx + dvz) + dvy) * c;  // compute_p! 3-D, advance_acou.jl:310-313
+    for (int q = 0; q < X.ninj; ++q) {
+        float acc;
+        int tt;
+        if (X.inj[q].kind == INJ_P && injection_sum(X.inj[q], b, o, X.it, acc, tt)) pn = pn + (acc * c);
+    }
+    P.s[0][o] = pn;
+}
+
+// ---------------------------------------------------------------------------------------
+// elastic
+// ---------------------------------------------------------------------------------------
+
+template <int O>
+__global__ __launch_bounds__(256) void vel3d_elastic(VelParams3D P, PassExtras X)
+{
+    constexpr int H = O - 1;
+    const int b = blockIdx.x;
+    const int nxw = P.nx - H;
+    if (b >= P.gx * P.gy * nxw) {
+        gather_block(X, b - P.gx * P.gy * nxw);
+        return;
+    }
+    Idx3 t;
+    decode(b, P.gx, P.gy, t);
+    const int i = t.i, j = t.j, k = t.k;
+    const int s1 = (int)P.s1, s2 = (int)P.s2;
+    const float *txx = P.s[0], *tyy = P.s[1], *tzz = P.s[2], *txy = P.s[3], *txz = P.s[4], *tyz = P.s[5];
+    const int o = (i + H) + s1 * (j + H) + s2 * (k + H);
+    const int c = i + s1 * j + s2 * k;
+    // vx -= dtinvavxirho*((dtauxxdx + dtauxydy) + dtauxzdz) on (nz-2h, ny-2h, nx-h)
+    if (i < P.nz - 2 * H && j < P.ny - 2 * H && k < P.nx - H) {
+        float a1 = dstride<O>(txx, (i + H) + s1 * (j + H) + s2 * k, s2) * P.dxI;   // D_xi(tauxx)
+        a1 = cpml3_x(a1, P.qx[0], i, j, k, P.npml, P.pml_mask);
+        float a2 = dstride<O>(txy, c, s1) * P.dyI;                                 // D_ya(tauxy)
+        a2 = cpml3_y(a2, P.qy[0], i, j, k, P.npml, P.pml_mask);
+        float a3 = dstride<O>(txz, c, 1) * P.dzI;                                  // D_za(tauxz)
+        a3 = cpml3_z(a3, P.qz[0], i, j, k, P.npml, P.pml_mask);
+        float nv = P.vx[o] - P.cx[c] * ((a1 + a2) + a3);
+        if ((P.rigid_mask & GPI_FACE(2, 0)) && k == 0) P.vx[(i + H) + s1 * (j + H)] = -nv;
+        if ((P.rigid_mask & GPI_FACE(2, 1)) && k == P.nx - H - 1)
+            P.vx[(i + H) + s1 * (j + H) + s2 * (P.nx + H - 1)] = -nv;
+        if (X.ninj) nv = inject_velocity(X, INJ_VX, b, o, nv, P.rho, P.dt);
+        P.vx[o] = nv;
+    }
+    // vy -= dtinvavyirho*((dtauxydx + dtauyydy) + dtauyzdz) on (nz-2h, ny-h, nx-2h)
+    if (i < P.nz - 2 * H && j < P.ny - H && k < P.nx - 2 * H) {
+        float a1 = dstride<O>(txy, c, s2) * P.dxI;                                 // D_xa(tauxy)
+        a1 = cpml3_x(a1, P.qx[1], i, j, k, P.npml, P.pml_mask);
+        float a2 = dstride<O>(tyy, (i + H) + s1 * j + s2 * (k + H), s1) * P.dyI;   // D_yi(tauyy)
+        a2 = cpml3_y(a2, P.qy[1], i, j, k, P.npml, P.pml_mask);
+        float a3 = dstride<O>(tyz, c, 1) * P.dzI;                                  // D_za(tauyz)
+        a3 = cpml3_z(a3, P.qz[1], i, j, k, P.npml, P.pml_mask);
+        float nv = P.vy[o] - P.cy[c] * ((a1 + a2) + a3);
+        if ((P.rigid_mask & GPI_FACE(1, 0)) && j == 0) P.vy[(i + H) + s2 * (k + H)] = -nv;
+        if ((P.rigid_mask & GPI_FACE(1, 1)) && j == P.ny - H - 1)
+            P.vy[(i + H) + s1 * (P.ny + H - 1) + s2 * (k + H)] = -nv;
+        if (X.ninj) nv = inject_velocity(X, INJ_VY, b, o, nv, P.rho, P.dt);
+        P.vy[o] = nv;
+    }
+    // vz -= dtinvavzirho*((dtauxzdx + dtauyzdy) + dtauzzdz) on (nz-h, ny-2h, nx-2h)
+    if (i < P.nz - H && j < P.ny - 2 * H && k < P.nx - 2 * H) {
+        float a1 = dstride<O>(txz, c, s2) * P.dxI;                                 // D_xa(tauxz)
+        a1 = cpml3_x(a1, P.qx[2], i, j, k, P.npml, P.pml_mask);
+        float a2 = dstride<O>(tyz, c, s1) * P.dyI;                                 // D_ya(tauyz)
+        a2 = cpml3_y(a2, P.qy[2], i, j, k, P.npml, P.pml_mask);
+        float a3 = dstride<O>(tzz, i + s1 * (j + H) + s2 * (k + H), 1) * P.dzI;   // D_zi(tauzz)
+        a3 = cpml3_z(a3, P.qz[2], i, j, k, P.npml, P.pml_mask);
+        float nv = P.vz[o] - P.cz[c] * ((a1 + a2) + a3);
+        if ((P.rigid_mask & GPI_FACE(0, 0)) && i == 0) P.vz[s1 * (j + H) + s2 * (k + H)] = -nv;
+        if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1)
+            P.vz[(P.nz + H - 1) + s1 * (j + H) + s2 * (k + H)] = -nv;
+        if (X.ninj) nv = inject_velocity(X, INJ_VZ, b, o, nv, P.rho, P.dt);
+        P.vz[o] = nv;
+    }
+}
+
+template <int O>
+__global__ __launch_bounds__(256) void str3d_elastic(StrParams3D P, PassExtras X)
+{
+    constexpr int H = O - 1;
+    const int b = blockIdx.x;
+    if (b >= P.gx * P.gy * P.nx) {
+        gather_block(X, b - P.gx * P.gy * P.nx);
+        return;
+    }
+    Idx3 t;
+    decode(b, P.gx, P.gy, t);
+    const int i = t.i, j = t.j, k = t.k;
+    const int s1 = (int)P.s1, s2 = (int)P.s2;
+    const bool fs = (P.free_mask & GPI_FACE(0, 0)) != 0;
+    const int o = i + s1 * j + s2 * k;
+    // normal stresses on the stress grid (compute_stressii!, advance_elastic.jl:155-177)
+    if (i < P.nz && j < P.ny) {
+        float dvx = dstride<O>(P.vx, o, s2) * P.dxI;    // D_xa(vx)
+        dvx = cpml3_x(dvx, P.qx[0], i, j, k, P.npml, P.pml_mask);
+        float dvy = dstride<O>(P.vy, o, s1) * P.dyI;    // D_ya(vy)
+        dvy = cpml3_y(dvy, P.qy[0], i, j, k, P.npml, P.pml_mask);
+        float dvz = dstride<O>(P.vz, o, 1) * P.dzI;     // D_za(vz)
+        dvz = cpml3_z(dvz, P.qz[0], i, j, k, P.npml, P.pml_mask);
+        const float m = P.c[0][o], l = P.c[1][o];
+        float txx = (P.s[0][o] - (m * dvx)) - (l * (dvy + dvz));
+        float tyy = (P.s[1][o] - (m * dvy)) - (l * (dvx + dvz));
+        float tzz = (P.s[2][o] - (m * dvz)) - (l * (dvy + dvx));
+        if (fs && i == 1) P.s[2][s1 * j + s2 * k] = -tzz;   // tauzz[1,:,:] = -tauzz[2,:,:]
+        for (int q = 0; q < X.ninj; ++q) {
+            float acc;
+            int tt;
+            if (X.inj[q].kind == INJ_NORMAL3 && injection_sum(X.inj[q], b, o, X.it, acc, tt)) {
+                txx = txx + (acc * m);
+                tyy = tyy + (acc * m);
+                tzz = tzz + (acc * m);
+            }
+        }
+        P.s[0][o] = txx;
+        P.s[1][o] = tyy;
+        if (!(fs && i == 0)) P.s[2][o] = tzz;
+    }
+    // tauxz -= dtavxzimu*(dvxdz + dvzdx) on (nz-h, ny-2h, nx-h)
+    if (i < P.nz - H && j < P.ny - 2 * H && k < P.nx - H) {
+        float a = dstride<O>(P.vx, i + s1 * (j + H) + s2 * (k + H), 1) * P.dzI;   // D_zi(vx)
+        a = cpml3_z(a, P.qz[1], i, j, k, P.npml, P.pml_mask);
+        float c = dstride<O>(P.vz, (i + H) + s1 * (j + H) + s2 * k, s2) * P.dxI;  // D_xi(vz)
+        c = cpml3_x(c, P.qx[2], i, j, k, P.npml, P.pml_mask);
+        const float v = P.s[4][o] - P.c[3][o] * (a + c);
+        P.s[4][o] = (fs && i == 0) ? 0.0f : v;
+    }
+    // tauxy -= dtavxyimu*(dvxdy + dvydx) on (nz-2h, ny-h, nx-h)
+    if (i < P.nz - 2 * H && j < P.ny - H && k < P.nx - H) {
+        float a = dstride<O>(P.vx, (i + H) + s1 * j + s2 * (k + H), s1) * P.dyI;  // D_yi(vx)
+        a = cpml3_y(a, P.qy[1], i, j, k, P.npml, P.pml_mask);
+        float c = dstride<O>(P.vy, (i + H) + s1 * (j + H) + s2 * k, s2) * P.dxI;  // D_xi(vy)
+        c = cpml3_x(c, P.qx[1], i, j, k, P.npml, P.pml_mask);
+        P.s[3][o] = P.s[3][o] - P.c[2][o] * (a + c);
+    }
+    // tauyz -= dtavyzimu*(dvydz + dvzdy) on (nz-h, ny-h, nx-2h)
+    if (i < P.nz - H && j < P.ny - H && k < P.nx - 2 * H) {
+        float a = dstride<O>(P.vy, i + s1 * (j + H) + s2 * (k + H), 1) * P.dzI;   // D_zi(vy)
+        a = cpml3_z(a, P.qz[2], i, j, k, P.npml, P.pml_mask);
+        float c = dstride<O>(P.vz, (i + H) + s1 * j + s2 * (k + H), s1) * P.dyI;  // D_yi(vz)
+        c = cpml3_y(c, P.qy[2], i, j, k, P.npml, P.pml_mask);
+        const float v = P.s[5][o] - P.c[4][o] * (a + c);
+        P.s[5][o] = (fs && i == 0) ? 0.0f : v;
+    }
+}
+
+template <int O>
+void launch_vel3(const Model& m, hipStream_t s, VelParams3D P, const PassExtras& X)
+{
+    constexpr int H = O - 1;
+    P.gx = (P.nz - H + 63) / 64;
+    P.gy = (P.ny - H + 3) / 4;
+    dim3 grd(P.gx * P.gy * (P.nx - H) + X.gather_blocks), blk(64, 4);
+    if (m.g.physics == 0) hipLaunchKernelGGL(vel3d_acoustic<O>, grd, blk, 0, s, P, X);
+    else hipLaunchKernelGGL(vel3d_elastic<O>, grd, blk, 0, s, P, X);
+}
+
+template <int O>
+void launch_str3(const Model& m, hipStream_t s, StrParams3D P, const PassExtras& X)
+{
+    P.gx = (P.nz + 63) / 64;
+    P.gy = (P.ny + 3) / 4;
+    dim3 grd(P.gx * P.gy * P.nx + X.gather_blocks), blk(64, 4);
+    if (m.g.physics == 0) hipLaunchKernelGGL(str3d_acoustic<O>, grd, blk, 0, s, P, X);
+    else hipLaunchKernelGGL(str3d_elastic<O>, grd, blk, 0, s, P, X);
+}
+
+}  // namespace
+
+int velocity_blocks_3d(const Model& m, int* gx, int* gy)
+{
+    const int H = m.h;
+    *gx = (int)((m.g.n[0] - H + 63) / 64);
+    *gy = (int)((m.g.n[1] - H + 3) / 4);
+    return *gx * *gy * (int)(m.g.n[2] - H);
+}
+
+int stress_blocks_3d(const Model& m, int* gx, int* gy)
+{
+    *gx = (int)((m.g.n[0] + 63) / 64);
+    *gy = (int)((m.g.n[1] + 3) / 4);
+    return *gx * *gy * (int)m.g.n[2];
+}
+
+void velocity_pass_3d(const Model& m, hipStream_t s, const PassExtras& X)
+{
+    VelParams3D P{};
+    const bool ac = m.g.physics == 0;
+    if (ac) {
+        P.s[0] = m.field[GPI_P];
+        P.qx[0] = m.pml[GPI_DPDX];
+        P.qy[0] = m.pml[GPI_DPDY];
+        P.qz[0] = m.pml[GPI_DPDZ];
+    } else {
+        const int sf[6] = {GPI_TAUXX, GPI_TAUYY, GPI_TAUZZ, GPI_TAUXY, GPI_TAUXZ, GPI_TAUYZ};
+        for (int q = 0; q < 6; ++q) P.s[q] = m.field[sf[q]];
+        P.qx[0] = m.pml[GPI_DTAUXXDX];
+        P.qy[0] = m.pml[GPI_DTAUXYDY];
+        P.qz[0] = m.pml[GPI_DTAUXZDZ];
+        P.qx[1] = m.pml[GPI_DTAUXYDX];
+        P.qy[1] = m.pml[GPI_DTAUYYDY];
+        P.qz[1] = m.pml[GPI_DTAUYZDZ];
+        P.qx[2] = m.pml[GPI_DTAUXZDX];
+        P.qy[2] = m.pml[GPI_DTAUYZDY];
+        P.qz[2] = m.pml[GPI_DTAUZZDZ];
+    }
+    P.vx = m.field[GPI_VX];
+    P.vy = m.field[GPI_VY];
+    P.vz = m.field[GPI_VZ];
+    P.cx = m.coef[GPI_C_DTINVAVXIRHO];
+    P.cy = m.coef[GPI_C_DTINVAVYIRHO];
+    P.cz = m.coef[GPI_C_DTINVAVZIRHO];
+    P.rho = m.coef[GPI_C_RHO];
+    P.nz = (int)m.g.n[0];
+    P.ny = (int)m.g.n[1];
+    P.nx = (int)m.g.n[2];
+    P.s1 = m.pitch0;
+    P.s2 = m.pitch0 * m.pitch1;
+    P.npml = m.g.npml;
+    P.pml_mask = m.g.pml_mask;
+    P.rigid_mask = m.g.rigid_mask;
+    P.dzI = m.g.dsI[0];
+    P.dyI = m.g.dsI[1];
+    P.dxI = m.g.dsI[2];
+    P.dt = m.g.dt;
+    switch (m.g.order) {
+    case 2: launch_vel3<2>(m, s, P, X); break;
+    case 4: launch_vel3<4>(m, s, P, X); break;
+    case 6: launch_vel3<6>(m, s, P, X); break;
+    default: launch_vel3<8>(m, s, P, X); break;
+    }
+}
+
+void stress_pass_3d(const Model& m, hipStream_t s, const PassExtras& X)
+{
+    StrParams3D P{};
+    const bool ac = m.g.physics == 0;
+    P.vx = m.field[GPI_VX];
+    P.vy = m.field[GPI_VY];
+    P.vz = m.field[GPI_VZ];
+    if (ac) {
+        P.s[0] = m.field[GPI_P];
+        P.c[0] = m.coef[GPI_C_DTK];
+        P.qx[0] = m.pml[GPI_DVXDX];
+        P.qy[0] = m.pml[GPI_DVYDY];
+        P.qz[0] = m.pml[GPI_DVZDZ];
+    } else {
+        const int sf[6] = {GPI_TAUXX, GPI_TAUYY, GPI_TAUZZ, GPI_TAUXY, GPI_TAUXZ, GPI_TAUYZ};
+        for (int q = 0; q < 6; ++q) P.s[q] = m.field[sf[q]];
+        P.c[0] = m.coef[GPI_C_DTM];
+        P.c[1] = m.coef[GPI_C_DTLAMBDA];
+        P.c[2] = m.coef[GPI_C_DTAVXYIMU];
+        P.c[3] = m.coef[GPI_C_DTAVXZIMU];
+        P.c[4] = m.coef[GPI_C_DTAVYZIMU];
+        P.qx[0] = m.pml[GPI_DVXDX];
+        P.qx[1] = m.pml[GPI_DVYDX];
+        P.qx[2] = m.pml[GPI_DVZDX];
+        P.qy[0] = m.pml[GPI_DVYDY];
+        P.qy[1] = m.pml[GPI_DVXDY];
+        P.qy[2] = m.pml[GPI_DVZDY];
+        P.qz[0] = m.pml[GPI_DVZDZ];
+        P.qz[1] = m.pml[GPI_DVXDZ];
+        P.qz[2] = m.pml[GPI_DVYDZ];
+    }
+    P.nz = (int)m.g.n[0];
+    P.ny = (int)m.g.n[1];
+    P.nx = (int)m.g.n[2];
+    P.s1 = m.pitch0;
+    P.s2 = m.pitch0 * m.pitch1;
+    P.npml = m.g.npml;
+    P.pml_mask = m.g.pml_mask;
+    P.free_mask = m.g.free_mask;
+    P.dzI = m.g.dsI[0];
+    P.dyI = m.g.dsI[1];
+    P.dxI = m.g.dsI[2];
+    switch (m.g.order) {
+    case 2: launch_str3<2>(m, s, P, X); break;
+    case 4: launch_str3<4>(m, s, P, X); break;
+    case 6: launch_str3<6>(m, s, P, X); break;
+    default: launch_str3<8>(m, s, P, X); break;
+    }
+}
+
+}  // namespace gpi

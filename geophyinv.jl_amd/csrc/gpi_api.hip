@@ -7,6 +7,7 @@
 #include <new>
 
 #include "fdtd2d.h"
+#include "fdtd3d.h"
 #include "gpi_internal.h"
 #include "pass_extras.h"
 
@@ -304,7 +305,6 @@ int gpi_model_create(gpi_ctx* c, const gpi_grid* g, const gpi_array* coefs, int3
 {
     if (!c || !g || !model_out) return fail(c, GPI_EINVAL, "null argument");
     if (g->ndim != 2 && g->ndim != 3) return fail(c, GPI_EINVAL, "ndim must be 2 or 3");
-    if (g->ndim == 3) return fail(c, GPI_EINVAL, "3-D grids are not supported by this build");
     if (g->order != 2 && g->order != 4 && g->order != 6 && g->order != 8)
         return fail(c, GPI_EINVAL, "order must be 2, 4, 6 or 8");
     if (g->physics != 0 && g->physics != 1) return fail(c, GPI_EINVAL, "physics must be 0 or 1");
@@ -481,8 +481,9 @@ int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
 
     // ---- sources: unique targets, grouped by the workgroup of the pass that owns them ----
     int vgx, vgy, sgx, sgy;
-    const int vnb = velocity_blocks_2d(m, &vgx, &vgy);
-    const int snb = stress_blocks_2d(m, &sgx, &sgy);
+    const bool d3 = g.ndim == 3;
+    const int vnb = d3 ? velocity_blocks_3d(m, &vgx, &vgy) : velocity_blocks_2d(m, &vgx, &vgy);
+    const int snb = d3 ? stress_blocks_3d(m, &sgx, &sgy) : stress_blocks_2d(m, &sgx, &sgy);
     std::vector<DevPoints> src(shot->nsrc);
     std::vector<int> kinds(shot->nsrc);
     bool vel_fused = true, str_fused = true;
@@ -518,14 +519,16 @@ int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
         std::vector<Tgt> tl;
         for (auto& kv : tg) {
             const int64_t lin = kv.first;
-            const int64_t iz = lin % n3[0], r = lin / n3[0], ix = r / n3[1];
+            const int64_t iz = lin % n3[0], r = lin / n3[0], iy = r % n3[1], ix = r / n3[1];
             int blk;
             if (vel) {
                 // muladd_with_density_v*! touches @inn(v) only (source.jl:166-177)
                 if (!(iz >= h && iz < n3[0] - h && ix >= h && ix < n3[2] - h)) continue;
-                blk = (int)((iz - h) / 64 + vgx * ((ix - h) / 4));
+                if (d3 && !(iy >= h && iy < n3[1] - h)) continue;
+                blk = d3 ? (int)((iz - h) / 64 + vgx * ((iy - h) / 4 + vgy * (ix - h)))
+                         : (int)((iz - h) / 64 + vgx * ((ix - h) / 4));
             } else {
-                blk = (int)(iz / 64 + sgx * (ix / 4));
+                blk = d3 ? (int)(iz / 64 + sgx * (iy / 4 + sgy * ix)) : (int)(iz / 64 + sgx * (ix / 4));
                 // a source on the free-surface row is applied after the surface condition
                 // by a separate kernel (source.jl:61-119 runs after update_stress!)
                 if ((g.free_mask & GPI_FACE(0, 0)) && iz == 0) str_fused = false;
@@ -685,11 +688,13 @@ int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
         XV.it = it;
         XS.it = it;
         if (samp) HIPTRY(c, hipEventRecord(E[0], c->stream));
-        velocity_pass_2d(m, c->stream, XV);                                          // :177-202
+        if (d3) velocity_pass_3d(m, c->stream, XV);                                  // :177-202
+        else velocity_pass_2d(m, c->stream, XV);
         if (samp) HIPTRY(c, hipEventRecord(E[1], c->stream));
         for (int q : vfall) inject(m, src[q], it, c->stream);
         if (samp) HIPTRY(c, hipEventRecord(E[2], c->stream));
-        stress_pass_2d(m, c->stream, XS);                                            // :208-223
+        if (d3) stress_pass_3d(m, c->stream, XS);                                    // :208-223
+        else stress_pass_2d(m, c->stream, XS);
         if (samp) HIPTRY(c, hipEventRecord(E[3], c->stream));
         for (int q : sfall) inject(m, src[q], it, c->stream);
         launches += 2 + (int64_t)vfall.size() + (int64_t)sfall.size();
@@ -698,7 +703,8 @@ int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
             next_sample = isamp < nsamp ? (isamp * nt) / nsamp : -1;
         }
     }
-    velocity_pass_2d(m, c->stream, X0);                                              // :254-255
+    if (d3) velocity_pass_3d(m, c->stream, X0);                                      // :254-255
+    else velocity_pass_2d(m, c->stream, X0);
     ++launches;
     HIPTRY(c, hipEventRecord(c->ev[1], c->stream));
     HIPTRY(c, hipGetLastError());

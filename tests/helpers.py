@@ -64,3 +64,37 @@ def experiment(medium, *, nt=240, sfields=("p",), rfields=("p", "vx", "vz"), ns=
     return G.SeisForwExpt(attrib, medium=medium, tgrid=tgrid, ageom=[ag], srcwav=[src],
                           rfields=list(rfields), order=order, npml=npml,
                           stressfree_faces=list(stressfree_faces), context=ctx, **kw)
+
+
+def medium3d(physics, n=(22, 18, 24), ds=10.0, seed=5):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    grid = [Grid(0.0, ds, n[0]), Grid(0.0, ds, n[1]), Grid(0.0, ds, n[2])]
+    vp = (2000.0 + 500.0 * (np.arange(n[0])[:, None, None] / n[0]) + 60.0 * rng.standard_normal(n)).astype(np.float32)
+    rho = (1800.0 + 300.0 * rng.random(n)).astype(np.float32)
+    if physics == 0:
+        return G.AcousticMedium(grid, vp, rho)
+    vs = (vp / (1.7 + 0.2 * rng.random(n))).astype(np.float32)
+    return G.ElasticMedium(grid, vp, vs, rho)
+
+
+def experiment3d(medium, *, nt=120, sfields=("p",), rfields=("p", "vx", "vy", "vz"), ns=2, nr=6,
+                 order=2, npml=5, pml_faces=None, stressfree_faces=(), compat=1, seed=9, ctx=None):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    gz, gy, gx = medium.grid
+    dt = 0.3 * gz.step / (1.7 * float(medium.vp.m.max()))
+    tgrid = Grid(0.0, dt, nt)
+    s = {"z": gz.last * rng.uniform(0.3, 0.6, ns), "y": gy.last * rng.uniform(0.3, 0.7, ns),
+         "x": gx.last * rng.uniform(0.3, 0.7, ns)}
+    r = {"z": gz.step * rng.uniform(0, 1.5, nr), "y": gy.last * rng.uniform(0, 1, nr),
+         "x": gx.last * rng.uniform(0, 1, nr)}
+    ag = G.AGeomss(s, r)
+    src = G.Srcs(ns, tgrid, list(sfields))
+    w = np.stack([wavelet(nt, dt, 18.0, 0.05 + 0.005 * k) for k in range(ns)], axis=1)
+    src.update(list(sfields), w)
+    attrib = G.FdtdAcoustic() if isinstance(medium, G.AcousticMedium) else G.FdtdElastic()
+    kw = {}
+    if pml_faces is not None:
+        kw["pml_faces"] = pml_faces
+    return G.SeisForwExpt(attrib, medium=medium, tgrid=tgrid, ageom=[ag], srcwav=[src],
+                          rfields=list(rfields), order=order, npml=npml, compat=compat,
+                          stressfree_faces=list(stressfree_faces), context=ctx, **kw)

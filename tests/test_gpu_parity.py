@@ -117,3 +117,39 @@ def test_medium_update_resident(gpu_ctx):
     r2, _ = pa.run_shot(0)
     assert rel_l2(r1["p"], r2["p"]) > 1e-3
     compare(pa, ["p"], "after medium update")
+
+
+# ------------------------------------------------------------------------------ 3-D
+from helpers import experiment3d, medium3d  # noqa: E402
+
+
+@pytest.mark.parametrize("order", [2, 4, 8])
+@pytest.mark.parametrize("compat", [1, 0])
+def test_acoustic_3d(gpu_ctx, order, compat):
+    n = (22, 18, 24) if order < 8 else (32, 30, 34)
+    pa = experiment3d(medium3d(0, n=n), order=order, sfields=("p", "vy"), compat=compat,
+                      ctx=gpu_ctx, seed=order + compat)
+    compare(pa, ["p", "vx", "vy", "vz"], f"acoustic 3-D order {order} compat {compat}")
+
+
+@pytest.mark.parametrize("order", [2, 4, 8])
+def test_elastic_3d(gpu_ctx, order):
+    n = (22, 18, 24) if order < 8 else (32, 30, 34)
+    pa = experiment3d(medium3d(1, n=n), order=order, sfields=("tauxx", "vx"), rfields=("vx", "vy", "vz"),
+                      ctx=gpu_ctx, seed=30 + order)
+    compare(pa, ["tauxx", "tauyy", "tauzz", "tauxy", "tauxz", "tauyz", "vx", "vy", "vz"],
+            f"elastic 3-D order {order}")
+
+
+def test_elastic_3d_free_surface(gpu_ctx):
+    pa = experiment3d(medium3d(1), order=2, sfields=("tauzz", "vz"), rfields=("vx", "vz"),
+                      pml_faces=["zmax", "ymin", "ymax", "xmin", "xmax"], stressfree_faces=["zmin"],
+                      ctx=gpu_ctx, seed=41)
+    compare(pa, ["tauzz", "tauxz", "tauyz", "vz"], "elastic 3-D free surface")
+
+
+def test_acoustic_3d_large_grid_short(gpu_ctx):
+    """A 160^3 medium (170^3 with CPML) for 12 steps."""
+    pa = experiment3d(medium3d(0, n=(160, 160, 160), ds=10.0), nt=12, npml=5, ns=1, nr=40,
+                      ctx=gpu_ctx, seed=3)
+    compare(pa, ["p", "vz"], "acoustic 3-D 170^3")
