@@ -44,7 +44,9 @@ struct StrParams2D {
 int velocity_blocks_2d(const Model& m, int* gx, int* gy);
 int stress_blocks_2d(const Model& m, int* gx, int* gy);
 
-void velocity_pass_2d(const Model& m, hipStream_t s, const PassExtras& X);
-void stress_pass_2d(const Model& m, hipStream_t s, const PassExtras& X);
+// The pass launchers return -1 without launching when the state lacks an array the pass
+// writes (a wavefield, or the CPML memory of an active derivative).
+int velocity_pass_2d(const Model& m, const State& st, hipStream_t s, const PassExtras& X);
+int stress_pass_2d(const Model& m, const State& st, hipStream_t s, const PassExtras& X);
 
 }  // namespace gpi

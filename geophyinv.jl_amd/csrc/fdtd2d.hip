@@ -285,22 +285,22 @@ int stress_blocks_2d(const Model& m, int* gx, int* gy)
     return *gx * *gy;
 }
 
-void velocity_pass_2d(const Model& m, hipStream_t s, const PassExtras& X)
+int velocity_pass_2d(const Model& m, const State& st, hipStream_t s, const PassExtras& X)
 {
     VelParams2D P{};
     const bool ac = m.g.physics == 0;
-    P.s0 = m.field[ac ? GPI_P : GPI_TAUXX];
-    P.s1 = ac ? nullptr : m.field[GPI_TAUZZ];
-    P.s2 = ac ? nullptr : m.field[GPI_TAUXZ];
-    P.vx = m.field[GPI_VX];
-    P.vz = m.field[GPI_VZ];
+    P.s0 = st.field[ac ? GPI_P : GPI_TAUXX];
+    P.s1 = ac ? nullptr : st.field[GPI_TAUZZ];
+    P.s2 = ac ? nullptr : st.field[GPI_TAUXZ];
+    P.vx = st.field[GPI_VX];
+    P.vz = st.field[GPI_VZ];
     P.cx = m.coef[GPI_C_DTINVAVXIRHO];
     P.cz = m.coef[GPI_C_DTINVAVZIRHO];
     P.rho = m.coef[GPI_C_RHO];
-    P.qx0 = m.pml[ac ? GPI_DPDX : GPI_DTAUXXDX];
-    P.qx1 = ac ? Pml{} : m.pml[GPI_DTAUXZDX];
-    P.qz0 = m.pml[ac ? GPI_DPDZ : GPI_DTAUZZDZ];
-    P.qz1 = ac ? Pml{} : m.pml[GPI_DTAUXZDZ];
+    P.qx0 = pml_of(m, st, ac ? GPI_DPDX : GPI_DTAUXXDX);
+    P.qx1 = ac ? Pml{} : pml_of(m, st, GPI_DTAUXZDX);
+    P.qz0 = pml_of(m, st, ac ? GPI_DPDZ : GPI_DTAUZZDZ);
+    P.qz1 = ac ? Pml{} : pml_of(m, st, GPI_DTAUXZDZ);
     P.nz = (int)m.g.n[0];
     P.nx = (int)m.g.n[2];
     P.ld = (int)m.pitch0;
@@ -310,30 +310,33 @@ void velocity_pass_2d(const Model& m, hipStream_t s, const PassExtras& X)
     P.dxI = m.g.dsI[2];
     P.dzI = m.g.dsI[0];
     P.dt = m.g.dt;
+    const Pml* q[4] = {&P.qx0, &P.qx1, &P.qz0, &P.qz1};
+    if (!state_complete(m, q, 4) || !P.s0 || !P.vx || !P.vz || (!ac && (!P.s1 || !P.s2))) return -1;
     switch (m.g.order) {
     case 2: launch_vel<2>(m, s, P, X); break;
     case 4: launch_vel<4>(m, s, P, X); break;
     case 6: launch_vel<6>(m, s, P, X); break;
     default: launch_vel<8>(m, s, P, X); break;
     }
+    return 0;
 }
 
-void stress_pass_2d(const Model& m, hipStream_t s, const PassExtras& X)
+int stress_pass_2d(const Model& m, const State& st, hipStream_t s, const PassExtras& X)
 {
     StrParams2D P{};
     const bool ac = m.g.physics == 0;
-    P.vx = m.field[GPI_VX];
-    P.vz = m.field[GPI_VZ];
-    P.s0 = m.field[ac ? GPI_P : GPI_TAUXX];
-    P.s1 = ac ? nullptr : m.field[GPI_TAUZZ];
-    P.s2 = ac ? nullptr : m.field[GPI_TAUXZ];
+    P.vx = st.field[GPI_VX];
+    P.vz = st.field[GPI_VZ];
+    P.s0 = st.field[ac ? GPI_P : GPI_TAUXX];
+    P.s1 = ac ? nullptr : st.field[GPI_TAUZZ];
+    P.s2 = ac ? nullptr : st.field[GPI_TAUXZ];
     P.c0 = m.coef[ac ? GPI_C_DTK : GPI_C_DTM];
     P.c1 = ac ? nullptr : m.coef[GPI_C_DTLAMBDA];
     P.c2 = ac ? nullptr : m.coef[GPI_C_DTAVMU];
-    P.qx0 = m.pml[GPI_DVXDX];
-    P.qx1 = ac ? Pml{} : m.pml[GPI_DVZDX];
-    P.qz0 = m.pml[GPI_DVZDZ];
-    P.qz1 = ac ? Pml{} : m.pml[GPI_DVXDZ];
+    P.qx0 = pml_of(m, st, GPI_DVXDX);
+    P.qx1 = ac ? Pml{} : pml_of(m, st, GPI_DVZDX);
+    P.qz0 = pml_of(m, st, GPI_DVZDZ);
+    P.qz1 = ac ? Pml{} : pml_of(m, st, GPI_DVXDZ);
     P.nz = (int)m.g.n[0];
     P.nx = (int)m.g.n[2];
     P.ld = (int)m.pitch0;
@@ -342,12 +345,15 @@ void stress_pass_2d(const Model& m, hipStream_t s, const PassExtras& X)
     P.free_mask = m.g.free_mask;
     P.dxI = m.g.dsI[2];
     P.dzI = m.g.dsI[0];
+    const Pml* q[4] = {&P.qx0, &P.qx1, &P.qz0, &P.qz1};
+    if (!state_complete(m, q, 4) || !P.s0 || !P.vx || !P.vz || (!ac && (!P.s1 || !P.s2))) return -1;
     switch (m.g.order) {
     case 2: launch_str<2>(m, s, P, X); break;
     case 4: launch_str<4>(m, s, P, X); break;
     case 6: launch_str<6>(m, s, P, X); break;
     default: launch_str<8>(m, s, P, X); break;
     }
+    return 0;
 }
 
 }  // namespace gpi

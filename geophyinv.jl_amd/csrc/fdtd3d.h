@@ -47,7 +47,7 @@ struct StrParams3D {
 
 int velocity_blocks_3d(const Model& m, int* gx, int* gy);
 int stress_blocks_3d(const Model& m, int* gx, int* gy);
-void velocity_pass_3d(const Model& m, hipStream_t s, const PassExtras& X);
-void stress_pass_3d(const Model& m, hipStream_t s, const PassExtras& X);
+int velocity_pass_3d(const Model& m, const State& st, hipStream_t s, const PassExtras& X);
+int stress_pass_3d(const Model& m, const State& st, hipStream_t s, const PassExtras& X);
 
 }  // namespace gpi

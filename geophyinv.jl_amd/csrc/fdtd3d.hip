@@ -340,31 +340,31 @@ int stress_blocks_3d(const Model& m, int* gx, int* gy)
     return *gx * *gy * (int)m.g.n[2];
 }
 
-void velocity_pass_3d(const Model& m, hipStream_t s, const PassExtras& X)
+int velocity_pass_3d(const Model& m, const State& st, hipStream_t s, const PassExtras& X)
 {
     VelParams3D P{};
     const bool ac = m.g.physics == 0;
     if (ac) {
-        P.s[0] = m.field[GPI_P];
-        P.qx[0] = m.pml[GPI_DPDX];
-        P.qy[0] = m.pml[GPI_DPDY];
-        P.qz[0] = m.pml[GPI_DPDZ];
+        P.s[0] = st.field[GPI_P];
+        P.qx[0] = pml_of(m, st, GPI_DPDX);
+        P.qy[0] = pml_of(m, st, GPI_DPDY);
+        P.qz[0] = pml_of(m, st, GPI_DPDZ);
     } else {
         const int sf[6] = {GPI_TAUXX, GPI_TAUYY, GPI_TAUZZ, GPI_TAUXY, GPI_TAUXZ, GPI_TAUYZ};
-        for (int q = 0; q < 6; ++q) P.s[q] = m.field[sf[q]];
-        P.qx[0] = m.pml[GPI_DTAUXXDX];
-        P.qy[0] = m.pml[GPI_DTAUXYDY];
-        P.qz[0] = m.pml[GPI_DTAUXZDZ];
-        P.qx[1] = m.pml[GPI_DTAUXYDX];
-        P.qy[1] = m.pml[GPI_DTAUYYDY];
-        P.qz[1] = m.pml[GPI_DTAUYZDZ];
-        P.qx[2] = m.pml[GPI_DTAUXZDX];
-        P.qy[2] = m.pml[GPI_DTAUYZDY];
-        P.qz[2] = m.pml[GPI_DTAUZZDZ];
+        for (int q = 0; q < 6; ++q) P.s[q] = st.field[sf[q]];
+        P.qx[0] = pml_of(m, st, GPI_DTAUXXDX);
+        P.qy[0] = pml_of(m, st, GPI_DTAUXYDY);
+        P.qz[0] = pml_of(m, st, GPI_DTAUXZDZ);
+        P.qx[1] = pml_of(m, st, GPI_DTAUXYDX);
+        P.qy[1] = pml_of(m, st, GPI_DTAUYYDY);
+        P.qz[1] = pml_of(m, st, GPI_DTAUYZDZ);
+        P.qx[2] = pml_of(m, st, GPI_DTAUXZDX);
+        P.qy[2] = pml_of(m, st, GPI_DTAUYZDY);
+        P.qz[2] = pml_of(m, st, GPI_DTAUZZDZ);
     }
-    P.vx = m.field[GPI_VX];
-    P.vy = m.field[GPI_VY];
-    P.vz = m.field[GPI_VZ];
+    P.vx = st.field[GPI_VX];
+    P.vy = st.field[GPI_VY];
+    P.vz = st.field[GPI_VZ];
     P.cx = m.coef[GPI_C_DTINVAVXIRHO];
     P.cy = m.coef[GPI_C_DTINVAVYIRHO];
     P.cz = m.coef[GPI_C_DTINVAVZIRHO];
@@ -381,44 +381,47 @@ void velocity_pass_3d(const Model& m, hipStream_t s, const PassExtras& X)
     P.dyI = m.g.dsI[1];
     P.dxI = m.g.dsI[2];
     P.dt = m.g.dt;
+    const Pml* q[9] = {&P.qx[0], &P.qx[1], &P.qx[2], &P.qy[0], &P.qy[1], &P.qy[2], &P.qz[0], &P.qz[1], &P.qz[2]};
+    if (!state_complete(m, q, 9) || !P.vx || !P.vy || !P.vz || !P.s[0]) return -1;
     switch (m.g.order) {
     case 2: launch_vel3<2>(m, s, P, X); break;
     case 4: launch_vel3<4>(m, s, P, X); break;
     case 6: launch_vel3<6>(m, s, P, X); break;
     default: launch_vel3<8>(m, s, P, X); break;
     }
+    return 0;
 }
 
-void stress_pass_3d(const Model& m, hipStream_t s, const PassExtras& X)
+int stress_pass_3d(const Model& m, const State& st, hipStream_t s, const PassExtras& X)
 {
     StrParams3D P{};
     const bool ac = m.g.physics == 0;
-    P.vx = m.field[GPI_VX];
-    P.vy = m.field[GPI_VY];
-    P.vz = m.field[GPI_VZ];
+    P.vx = st.field[GPI_VX];
+    P.vy = st.field[GPI_VY];
+    P.vz = st.field[GPI_VZ];
     if (ac) {
-        P.s[0] = m.field[GPI_P];
+        P.s[0] = st.field[GPI_P];
         P.c[0] = m.coef[GPI_C_DTK];
-        P.qx[0] = m.pml[GPI_DVXDX];
-        P.qy[0] = m.pml[GPI_DVYDY];
-        P.qz[0] = m.pml[GPI_DVZDZ];
+        P.qx[0] = pml_of(m, st, GPI_DVXDX);
+        P.qy[0] = pml_of(m, st, GPI_DVYDY);
+        P.qz[0] = pml_of(m, st, GPI_DVZDZ);
     } else {
         const int sf[6] = {GPI_TAUXX, GPI_TAUYY, GPI_TAUZZ, GPI_TAUXY, GPI_TAUXZ, GPI_TAUYZ};
-        for (int q = 0; q < 6; ++q) P.s[q] = m.field[sf[q]];
+        for (int q = 0; q < 6; ++q) P.s[q] = st.field[sf[q]];
         P.c[0] = m.coef[GPI_C_DTM];
         P.c[1] = m.coef[GPI_C_DTLAMBDA];
         P.c[2] = m.coef[GPI_C_DTAVXYIMU];
         P.c[3] = m.coef[GPI_C_DTAVXZIMU];
         P.c[4] = m.coef[GPI_C_DTAVYZIMU];
-        P.qx[0] = m.pml[GPI_DVXDX];
-        P.qx[1] = m.pml[GPI_DVYDX];
-        P.qx[2] = m.pml[GPI_DVZDX];
-        P.qy[0] = m.pml[GPI_DVYDY];
-        P.qy[1] = m.pml[GPI_DVXDY];
-        P.qy[2] = m.pml[GPI_DVZDY];
-        P.qz[0] = m.pml[GPI_DVZDZ];
-        P.qz[1] = m.pml[GPI_DVXDZ];
-        P.qz[2] = m.pml[GPI_DVYDZ];
+        P.qx[0] = pml_of(m, st, GPI_DVXDX);
+        P.qx[1] = pml_of(m, st, GPI_DVYDX);
+        P.qx[2] = pml_of(m, st, GPI_DVZDX);
+        P.qy[0] = pml_of(m, st, GPI_DVYDY);
+        P.qy[1] = pml_of(m, st, GPI_DVXDY);
+        P.qy[2] = pml_of(m, st, GPI_DVZDY);
+        P.qz[0] = pml_of(m, st, GPI_DVZDZ);
+        P.qz[1] = pml_of(m, st, GPI_DVXDZ);
+        P.qz[2] = pml_of(m, st, GPI_DVYDZ);
     }
     P.nz = (int)m.g.n[0];
     P.ny = (int)m.g.n[1];
@@ -431,12 +434,15 @@ void stress_pass_3d(const Model& m, hipStream_t s, const PassExtras& X)
     P.dzI = m.g.dsI[0];
     P.dyI = m.g.dsI[1];
     P.dxI = m.g.dsI[2];
+    const Pml* q[9] = {&P.qx[0], &P.qx[1], &P.qx[2], &P.qy[0], &P.qy[1], &P.qy[2], &P.qz[0], &P.qz[1], &P.qz[2]};
+    if (!state_complete(m, q, 9) || !P.vx || !P.vy || !P.vz || !P.s[0]) return -1;
     switch (m.g.order) {
     case 2: launch_str3<2>(m, s, P, X); break;
     case 4: launch_str3<4>(m, s, P, X); break;
     case 6: launch_str3<6>(m, s, P, X); break;
     default: launch_str3<8>(m, s, P, X); break;
     }
+    return 0;
 }
 
 }  // namespace gpi

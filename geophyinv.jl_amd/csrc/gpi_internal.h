@@ -10,15 +10,6 @@
 
 namespace gpi {
 
-// Device field view: element (iz, iy, ix) at p[iz + s1*iy + s2*ix].  All wavefields and
-// coefficient arrays of a model share the same pitches, so one offset addresses every
-// array; CPML memory arrays carry their own pitches.
-struct View {
-    float* p = nullptr;
-    int32_t n[3] = {0, 0, 0};   // logical sizes z, y, x
-    int64_t s1 = 0, s2 = 0;     // strides of y and x in floats
-};
-
 // CPML state of one derivative field: its memory array and the a, b, kI vectors.
 struct Pml {
     float* psi = nullptr;       // memory variables, axis dimension = 2*npml
@@ -42,13 +33,21 @@ struct DevPoints {
     float* w = nullptr;         // gather: npts*nnz weights
     int32_t ntarget = 0;        // scatter: unique target cells (interior only for velocity)
     int32_t* blk = nullptr;     // scatter: per-workgroup target ranges of the owning pass
-    bool fused = false;         // scatter: applied inside the owning pass
-    int64_t* toff = nullptr;    // scatter: padded offsets of targets
+    int64_t* toff = nullptr;    // scatter: padded offsets of targets (then 2 rho offsets each)
     int32_t* tptr = nullptr;    // scatter: ntarget+1 CSR row pointers
     int32_t* tsrc = nullptr;    // scatter: source column per contribution
     float* tw = nullptr;        // scatter: weight per contribution
     float* wav = nullptr;       // scatter: nt x npts wavelets, point fastest
     float* rec = nullptr;       // gather: nt x npts records, point fastest
+};
+
+// Wavefields and CPML memory of one propagating wavefield.  A model can serve several
+// states (two in the adjoint run, many in batched serving).
+struct State {
+    float* field[GPI_NFIELDS] = {};   // wavefields only (no derivative arrays)
+    float* psi[GPI_NFIELDS] = {};     // CPML memory of derivative fields
+    std::vector<void*> allocs;
+    uint64_t bytes = 0;
 };
 
 struct Model {
@@ -57,10 +56,24 @@ struct Model {
     int64_t pitch0 = 0, pitch1 = 1;    // wavefield/coefficient pitches (z, y)
     int64_t slab = 0;                  // floats per array (pitch0*pitch1*(nx+h))
     std::vector<void*> allocs;
+    uint64_t bytes = 0;
     float* coef[GPI_NCOEF] = {};
-    float* field[GPI_NFIELDS] = {};    // wavefields only (no derivative arrays)
-    Pml pml[GPI_NFIELDS];
+    Pml pml[GPI_NFIELDS];              // strides and coefficient vectors; psi lives in a State
     float* pmlvec = nullptr;           // all a, b, kI vectors
+    State* st0 = nullptr;              // the state of gpi_forward / gpi_forward_save
+};
+
+// Time-reversal store of one :forward_save shot (src/fdtd/boundary.jl): nt strips of the
+// stored stress fields per axis, plus the final snapshots.
+struct Store {
+    Model* model = nullptr;
+    int64_t nt = 0;
+    int nfield = 0;
+    int field[3] = {};                 // stored stress fields
+    float* strip[3][3] = {};           // [field][axis]: nt x strip_size
+    int64_t strip_size[3][3] = {};
+    float* snap[GPI_NFIELDS] = {};     // pitched snapshots: -stress at nt, v at nt+1/2
+    std::vector<void*> allocs;
     uint64_t bytes = 0;
 };
 
@@ -72,11 +85,36 @@ struct Ctx {
     uint64_t budget = 0, used = 0;
     std::string err;
     std::vector<Model*> models;
+    std::vector<Store*> stores;
     gpi_timing timing{};
 };
 
 // stand-alone sparse kernels (sparse.hip): injection after a pass, for source sets that
 // cannot ride along with the pass (targets on a free-surface row or dense target lists)
-void inject(const Model& m, const DevPoints& pts, int64_t it, hipStream_t s);
+void inject(const Model& m, const State& st, const DevPoints& pts, int64_t it, hipStream_t s);
+
+// time-reversal kernels (strips.hip)
+void strip_copy(const Model& m, float* field, const int64_t fn[3], int axis, int64_t np, float* store,
+                bool save, hipStream_t s);
+void negate_copy(float* dst, const float* src, int64_t n, hipStream_t s);
+// cur/prev/next: p1, vx1, vz1, p2, vx2, vz2 (pitched slabs); next <- cur after imaging
+void image_acoustic_2d(const Model& m, const float* const cur[6], const float* const prev[6], float* const next[6],
+                       float* g_invK, float* g_rho, float dtI, hipStream_t s);
+
+// CPML strides and vectors of derivative f with the memory array of state st
+inline Pml pml_of(const Model& m, const State& st, int f)
+{
+    Pml q = m.pml[f];
+    q.psi = st.psi[f];
+    return q;
+}
+
+// every active derivative of a CPML model has its memory array
+inline bool state_complete(const Model& m, const Pml* const* q, int n)
+{
+    for (int k = 0; k < n; ++k)
+        if (q[k]->active && m.g.pml_mask && !q[k]->psi) return false;
+    return true;
+}
 
 }  // namespace gpi

@@ -1,0 +1,590 @@
+// C-ABI of libgpi: the time loops of one supersource (forward, forward with boundary saving,
+// and the adjoint run with the reference imaging condition).
+#include <algorithm>
+#include <map>
+
+#include "fdtd2d.h"
+#include "fdtd3d.h"
+#include "gpi_util.h"
+#include "pass_extras.h"
+
+using namespace gpi;
+
+namespace {
+
+constexpr int NBOUND = 3;   // _fd_nbound (src/GeoPhyInv.jl:92)
+
+bool recordable(int f)
+{
+    // propagate.jl:177 records [:p] and :208 records [:vx,:vy,:vz]; other fields of
+    // rfields keep zero records, as in the reference.
+    return f == GPI_P || f == GPI_VX || f == GPI_VY || f == GPI_VZ;
+}
+
+int velocity_pass(gpi_ctx* c, const Model& m, const State& st, const PassExtras& X)
+{
+    const int rc = m.g.ndim == 3 ? velocity_pass_3d(m, st, c->stream, X) : velocity_pass_2d(m, st, c->stream, X);
+    return rc ? fail(c, GPI_EINVAL, "internal error: incomplete wavefield state") : GPI_OK;
+}
+
+int stress_pass(gpi_ctx* c, const Model& m, const State& st, const PassExtras& X)
+{
+    const int rc = m.g.ndim == 3 ? stress_pass_3d(m, st, c->stream, X) : stress_pass_2d(m, st, c->stream, X);
+    return rc ? fail(c, GPI_EINVAL, "internal error: incomplete wavefield state") : GPI_OK;
+}
+
+// Source sets of one wavefield: fused into the passes' PassExtras when every set of a pass
+// can ride along, else applied by the stand-alone kernel after the pass (all-or-nothing per
+// pass keeps the reference's order of updates).
+struct Sources {
+    std::vector<DevPoints> pts;
+    std::vector<int> vfall, sfall;   // sets injected after the velocity / stress pass
+};
+
+int plan_sources(gpi_ctx* c, const Model& m, Scratch& S, int nsrc, const gpi_points* src, const float* const* wav,
+                 int64_t nt, bool velocity_only, PassExtras& XV, PassExtras& XS, Sources& out)
+{
+    const gpi_grid& g = m.g;
+    const int nnz_expect = g.ndim == 2 ? 4 : 8;
+    int vgx, vgy, sgx, sgy;
+    const bool d3 = g.ndim == 3;
+    const int vnb = d3 ? velocity_blocks_3d(m, &vgx, &vgy) : velocity_blocks_2d(m, &vgx, &vgy);
+    const int snb = d3 ? stress_blocks_3d(m, &sgx, &sgy) : stress_blocks_2d(m, &sgx, &sgy);
+    out.pts.assign(nsrc, DevPoints{});
+    std::vector<int> kinds(nsrc);
+    bool vel_fused = true, str_fused = true;
+    for (int q = 0; q < nsrc; ++q) {
+        const gpi_points& P = src[q];
+        const int f = P.field;
+        if (!field_codes(g.ndim, g.physics, f) || !is_wavefield(f))
+            return fail(c, GPI_EINVAL, "source field " + std::to_string(f) + " is not a wavefield of this physics");
+        const bool vel = is_velocity(f);
+        const bool str = f == GPI_P || f == GPI_TAUXX || f == GPI_TAUYY || f == GPI_TAUZZ;
+        if (!vel && !str) return fail(c, GPI_EINVAL, "shear stresses cannot carry sources");
+        if (velocity_only && !vel) return fail(c, GPI_EINVAL, "adjoint sources must be velocity fields");
+        if (P.nnz != nnz_expect || P.npts < 0 || (P.npts > 0 && (!P.idx || !P.w || !wav || !wav[q])))
+            return fail(c, GPI_EINVAL, "bad source point set");
+        kinds[q] = f == GPI_VX ? INJ_VX : f == GPI_VY ? INJ_VY : f == GPI_VZ ? INJ_VZ
+                 : g.physics == 0 ? INJ_P : (d3 ? INJ_NORMAL3 : INJ_NORMAL2);
+        int64_t n3[3];
+        shape_of(g.ndim, g.order, g.physics, g.n, f, n3);
+        const int64_t N = n3[0] * n3[1] * n3[2];
+        const int h = m.h;
+        // target -> contributions in ascending source order
+        std::map<int64_t, std::vector<std::pair<int, float>>> tg;
+        for (int s = 0; s < P.npts; ++s)
+            for (int k = 0; k < P.nnz; ++k) {
+                const int64_t lin = P.idx[(size_t)s * P.nnz + k];
+                if (lin < 0 || lin >= N) return fail(c, GPI_EINVAL, "source index out of range");
+                tg[lin].push_back({s, P.w[(size_t)s * P.nnz + k]});
+            }
+        struct Tgt {
+            int64_t lin;
+            int blk;
+            const std::vector<std::pair<int, float>>* con;
+        };
+        std::vector<Tgt> tl;
+        for (auto& kv : tg) {
+            const int64_t lin = kv.first;
+            const int64_t iz = lin % n3[0], r = lin / n3[0], iy = r % n3[1], ix = r / n3[1];
+            int blk;
+            if (vel) {
+                // muladd_with_density_v*! touches @inn(v) only (source.jl:166-177)
+                if (!(iz >= h && iz < n3[0] - h && ix >= h && ix < n3[2] - h)) continue;
+                if (d3 && !(iy >= h && iy < n3[1] - h)) continue;
+                blk = d3 ? (int)((iz - h) / 64 + vgx * ((iy - h) / 4 + vgy * (ix - h)))
+                         : (int)((iz - h) / 64 + vgx * ((ix - h) / 4));
+            } else {
+                blk = d3 ? (int)(iz / 64 + sgx * (iy / 4 + sgy * ix)) : (int)(iz / 64 + sgx * (ix / 4));
+                // a source on the free-surface row is applied after the surface condition
+                // by the stand-alone kernel (source.jl:61-119 runs after update_stress!)
+                if ((g.free_mask & GPI_FACE(0, 0)) && iz == 0) str_fused = false;
+            }
+            tl.push_back({lin, blk, &kv.second});
+        }
+        std::stable_sort(tl.begin(), tl.end(), [](const Tgt& x, const Tgt& y) { return x.blk < y.blk; });
+        const int nb = vel ? vnb : snb;
+        std::vector<int32_t> blk(nb + 1, 0);
+        for (const Tgt& t : tl) blk[t.blk + 1]++;
+        int maxper = 0;
+        for (int k = 0; k < nb; ++k) {
+            maxper = std::max(maxper, blk[k + 1]);
+            blk[k + 1] += blk[k];
+        }
+        if (maxper > 32) (vel ? vel_fused : str_fused) = false;   // keep owner scans short
+        DevPoints& D = out.pts[q];
+        D.field = f;
+        D.npts = P.npts;
+        D.nnz = P.nnz;
+        std::vector<int64_t> toff, roff;
+        std::vector<int32_t> tptr{0}, tsrc;
+        std::vector<float> tw;
+        for (const Tgt& t : tl) {
+            if (vel) {
+                // @av_{x,y,z}i(rho) at the interior point: rho indices of the two samples
+                const int64_t iz = t.lin % n3[0], r = t.lin / n3[0], iy = r % n3[1], ix = r / n3[1];
+                int64_t b0[3] = {iz, iy, ix};
+                const int ax = f == GPI_VZ ? 0 : f == GPI_VY ? 1 : 2;
+                b0[ax] -= h;
+                int64_t b1[3] = {b0[0], b0[1], b0[2]};
+                b1[ax] += 1;
+                roff.push_back(b0[0] + m.pitch0 * (b0[1] + m.pitch1 * b0[2]));
+                roff.push_back(b1[0] + m.pitch0 * (b1[1] + m.pitch1 * b1[2]));
+            }
+            toff.push_back(padded(m, n3, t.lin));
+            for (auto& pr : *t.con) {
+                tsrc.push_back(pr.first);
+                tw.push_back(pr.second);
+            }
+            tptr.push_back((int32_t)tsrc.size());
+        }
+        D.ntarget = (int32_t)toff.size();
+        toff.insert(toff.end(), roff.begin(), roff.end());
+        int rc;
+        if ((rc = S.put(&D.toff, toff)) || (rc = S.put(&D.tptr, tptr)) || (rc = S.put(&D.tsrc, tsrc)) ||
+            (rc = S.put(&D.tw, tw)) || (rc = S.put(&D.blk, blk)))
+            return rc;
+        // wavelets: host nt x npts time fastest -> device point fastest per step
+        std::vector<float> wv((size_t)nt * std::max(P.npts, 1), 0.0f);
+        for (int s = 0; s < P.npts; ++s)
+            for (int64_t it = 0; it < nt; ++it) wv[(size_t)it * P.npts + s] = wav[q][(size_t)s * nt + it];
+        if ((rc = S.put(&D.wav, wv))) return rc;
+    }
+    for (int q = 0; q < nsrc; ++q) {
+        const DevPoints& D = out.pts[q];
+        if (D.ntarget == 0) continue;
+        const bool vel = kinds[q] >= INJ_VX;
+        PassExtras& X = vel ? XV : XS;
+        bool& fused = vel ? vel_fused : str_fused;
+        if (fused && X.ninj == MAX_INJ) fused = false;
+        if (fused) {
+            InjSet& I = X.inj[X.ninj++];
+            I.kind = kinds[q];
+            I.npts = D.npts;
+            I.blk = D.blk;
+            I.toff = D.toff;
+            I.tptr = D.tptr;
+            I.tsrc = D.tsrc;
+            I.tw = D.tw;
+            I.wav = D.wav;
+            I.roff = D.toff + D.ntarget;
+        }
+    }
+    if (!vel_fused) XV.ninj = 0;
+    if (!str_fused) XS.ninj = 0;
+    for (int q = 0; q < nsrc; ++q) {
+        if (out.pts[q].ntarget == 0) continue;
+        const bool vel = kinds[q] >= INJ_VX;
+        if (vel && !vel_fused) out.vfall.push_back(q);
+        if (!vel && !str_fused) out.sfall.push_back(q);
+    }
+    return GPI_OK;
+}
+
+// Receivers in gather form; p is sampled by the velocity pass (before it updates v), v by
+// the stress pass (after v's sources).
+int plan_receivers(gpi_ctx* c, const Model& m, const State& st, Scratch& S, const gpi_shot* shot, PassExtras& XV,
+                   PassExtras& XS, std::vector<DevPoints>& rec)
+{
+    const gpi_grid& g = m.g;
+    const int64_t nt = shot->nt;
+    const int nnz_expect = g.ndim == 2 ? 4 : 8;
+    rec.assign(shot->nrec, DevPoints{});
+    int vmax = 0, smax = 0;
+    for (int q = 0; q < shot->nrec; ++q) {
+        const gpi_points& P = shot->rec[q];
+        const int f = P.field;
+        if (!field_codes(g.ndim, g.physics, f) || !is_wavefield(f))
+            return fail(c, GPI_EINVAL, "receiver field " + std::to_string(f) + " is not a wavefield of this physics");
+        if (P.nnz != nnz_expect || P.npts < 0 || (P.npts > 0 && (!P.idx || !P.w)) || !shot->records ||
+            !shot->records[q])
+            return fail(c, GPI_EINVAL, "bad receiver point set");
+        int64_t n3[3];
+        shape_of(g.ndim, g.order, g.physics, g.n, f, n3);
+        const int64_t N = n3[0] * n3[1] * n3[2];
+        DevPoints& D = rec[q];
+        D.field = f;
+        D.npts = P.npts;
+        D.nnz = P.nnz;
+        std::vector<int64_t> off((size_t)P.npts * P.nnz);
+        for (size_t k = 0; k < off.size(); ++k) {
+            if (P.idx[k] < 0 || P.idx[k] >= N) return fail(c, GPI_EINVAL, "receiver index out of range");
+            off[k] = padded(m, n3, P.idx[k]);
+        }
+        std::vector<float> w(P.w, P.w + (size_t)P.npts * P.nnz);
+        int rc;
+        if ((rc = S.put(&D.idx, off)) || (rc = S.put(&D.w, w))) return rc;
+        if ((rc = S.alloc(&D.rec, (size_t)nt * std::max(P.npts, 1)))) return rc;
+        HIPTRY(c, hipMemsetAsync(D.rec, 0, sizeof(float) * (size_t)nt * std::max(P.npts, 1), c->stream));
+        if (!recordable(f) || P.npts == 0) continue;
+        const bool vfield = is_velocity(f);
+        GatherSet& G = vfield ? XS.g : XV.g;
+        if (G.nset == 4) return fail(c, GPI_EINVAL, "too many receiver fields");
+        const int k = G.nset++;
+        G.nnz = P.nnz;
+        G.npts[k] = P.npts;
+        G.field[k] = st.field[f];
+        G.idx[k] = D.idx;
+        G.w[k] = D.w;
+        G.rec[k] = D.rec;
+        int& mx = vfield ? smax : vmax;
+        mx = std::max(mx, P.npts);
+    }
+    XV.gather_blocks = XV.g.nset * ((vmax + 255) / 256);
+    XS.gather_blocks = XS.g.nset * ((smax + 255) / 256);
+    return GPI_OK;
+}
+
+// stress fields kept by boundary_save! (boundary.jl:217-264): p, or tauxx, tauxz, tauzz
+int stored_fields(const Model& m, int out[3])
+{
+    if (m.g.physics == 0) {
+        out[0] = GPI_P;
+        return 1;
+    }
+    out[0] = GPI_TAUXX;
+    out[1] = GPI_TAUXZ;
+    out[2] = GPI_TAUZZ;
+    return 3;
+}
+
+// boundary_{save,force}{x,y,z}! use the min-face flag of the axis for both strips
+// (boundary.jl:24,40)
+int64_t strip_np(const Model& m, int axis) { return (m.g.pml_mask & GPI_FACE(axis, 0)) ? m.g.npml : 0; }
+
+int alloc_store(gpi_ctx* c, Model& m, int64_t nt, Store** out)
+{
+    const gpi_grid& g = m.g;
+    if (g.physics == 1 && g.ndim == 3) return fail(c, GPI_EINVAL, "boundary saving of 3-D elastic is not in the reference");
+    Store* s = new Store();
+    s->model = &m;
+    s->nt = nt;
+    s->nfield = stored_fields(m, s->field);
+    auto bail = [&](int rc) {
+        free_allocs(c, s->allocs, s->bytes);
+        delete s;
+        return rc;
+    };
+    for (int q = 0; q < s->nfield; ++q) {
+        int64_t fn[3];
+        shape_of(g.ndim, g.order, g.physics, g.n, s->field[q], fn);
+        for (int ax = 0; ax < 3; ++ax) {
+            if (g.ndim == 2 && ax == 1) continue;
+            if (fn[ax] < strip_np(m, ax) + 2 * NBOUND)
+                return bail(fail(c, GPI_EINVAL, "grid too short for the boundary strips"));
+            int64_t sd[3] = {fn[0], fn[1], fn[2]};
+            sd[ax] = 2 * NBOUND;
+            s->strip_size[q][ax] = sd[0] * sd[1] * sd[2];
+            int rc = dev_alloc(c, &s->allocs, &s->bytes, (void**)&s->strip[q][ax],
+                               sizeof(float) * (size_t)(s->strip_size[q][ax] * nt));
+            if (rc) return bail(rc);
+        }
+        int rc = dev_alloc(c, &s->allocs, &s->bytes, (void**)&s->snap[s->field[q]], sizeof(float) * (size_t)m.slab);
+        if (rc) return bail(rc);
+    }
+    for (int f : {GPI_VX, GPI_VY, GPI_VZ}) {
+        if (!field_codes(g.ndim, g.physics, f)) continue;
+        int rc = dev_alloc(c, &s->allocs, &s->bytes, (void**)&s->snap[f], sizeof(float) * (size_t)m.slab);
+        if (rc) return bail(rc);
+    }
+    *out = s;
+    return GPI_OK;
+}
+
+// boundary_save! (save) or boundary_force! (!save) of step `it`: axes x, (y,) z
+void boundary_strips(const Model& m, const State& st, const Store& s, int64_t it, bool save, hipStream_t stream,
+                     int64_t* launches)
+{
+    for (int q = 0; q < s.nfield; ++q) {
+        int64_t fn[3];
+        shape_of(m.g.ndim, m.g.order, m.g.physics, m.g.n, s.field[q], fn);
+        for (int ax = 2; ax >= 0; --ax) {
+            if (!s.strip[q][ax]) continue;
+            strip_copy(m, st.field[s.field[q]], fn, ax, strip_np(m, ax), s.strip[q][ax] + s.strip_size[q][ax] * it,
+                       save, stream);
+            ++*launches;
+        }
+    }
+}
+
+Store* find_store(gpi_ctx* c, uint64_t h)
+{
+    for (Store* s : c->stores)
+        if ((uint64_t)(uintptr_t)s == h) return s;
+    return nullptr;
+}
+
+// Pass timing of the sampled steps.
+struct Sampler {
+    gpi_ctx* c;
+    int64_t nt, nsamp, next, k = 0;
+    explicit Sampler(gpi_ctx* cc, int64_t n) : c(cc), nt(n)
+    {
+        nsamp = std::min<int64_t>((int64_t)c->sample_ev.size() / 4, nt);
+        next = nsamp ? 0 : -1;
+    }
+    hipEvent_t* at(int64_t it) { return it == next ? &c->sample_ev[4 * k] : nullptr; }
+    void done(int64_t it)
+    {
+        if (it != next) return;
+        ++k;
+        next = k < nsamp ? (k * nt) / nsamp : -1;
+    }
+};
+
+int finish_timing(gpi_ctx* c, const Sampler& smp, int64_t nt, int64_t launches)
+{
+    HIPTRY(c, hipEventRecord(c->ev[1], c->stream));
+    HIPTRY(c, hipGetLastError());
+    HIPTRY(c, hipEventSynchronize(c->ev[1]));
+    float ms = 0.0f;
+    HIPTRY(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->timing = gpi_timing{};
+    c->timing.total_ms = ms;
+    c->timing.steps = nt;
+    c->timing.launches = launches;
+    c->timing.sampled = smp.k;
+    double vs = 0.0, ss = 0.0;
+    for (int64_t k = 0; k < smp.k; ++k) {
+        float a = 0.0f, b = 0.0f;
+        HIPTRY(c, hipEventElapsedTime(&a, c->sample_ev[4 * k], c->sample_ev[4 * k + 1]));
+        HIPTRY(c, hipEventElapsedTime(&b, c->sample_ev[4 * k + 2], c->sample_ev[4 * k + 3]));
+        vs += a;
+        ss += b;
+    }
+    if (smp.k) {
+        c->timing.velocity_avg_ms = vs / (double)smp.k;
+        c->timing.stress_avg_ms = ss / (double)smp.k;
+    }
+    return GPI_OK;
+}
+
+// mod_x_proc! for one supersource in :forward (save == nullptr) or :forward_save mode
+// (src/fdtd/propagate.jl:138-261).
+int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
+{
+    const gpi_grid& g = m.g;
+    const int64_t nt = shot->nt;
+    const State& st = *m.st0;
+    Scratch S(c);
+    PassExtras XV{}, XS{}, X0{};
+    Sources src;
+    int rc = plan_sources(c, m, S, shot->nsrc, shot->src, shot->wavelets, nt, false, XV, XS, src);
+    if (rc) return rc;
+    std::vector<DevPoints> rec;
+    if ((rc = plan_receivers(c, m, st, S, shot, XV, XS, rec))) return rc;
+    for (int q = 0; q < shot->nfinal; ++q) {
+        const int f = shot->final_fields[q];
+        if (f < 0 || f >= GPI_NFIELDS || !is_wavefield(f) || !st.field[f] || !shot->final_out[q])
+            return fail(c, GPI_EINVAL, "bad final field request");
+    }
+
+    // reset wavefields and CPML memory (reset_w2!, types.jl:101-113)
+    if ((rc = zero_state(c, m, st, c->stream))) return rc;
+
+    // per step: velocity pass (records p at :177, sources of :202 ride along), then stress
+    // pass (records v at :208, sources of :223 ride along), then boundary_save! (:229)
+    Sampler smp(c, nt);
+    int64_t launches = 0;
+    HIPTRY(c, hipEventRecord(c->ev[0], c->stream));
+    for (int64_t it = 0; it < nt; ++it) {
+        hipEvent_t* E = smp.at(it);
+        XV.it = it;
+        XS.it = it;
+        if (E) HIPTRY(c, hipEventRecord(E[0], c->stream));
+        if ((rc = velocity_pass(c, m, st, XV))) return rc;
+        if (E) HIPTRY(c, hipEventRecord(E[1], c->stream));
+        for (int q : src.vfall) inject(m, st, src.pts[q], it, c->stream);
+        if (E) HIPTRY(c, hipEventRecord(E[2], c->stream));
+        if ((rc = stress_pass(c, m, st, XS))) return rc;
+        if (E) HIPTRY(c, hipEventRecord(E[3], c->stream));
+        for (int q : src.sfall) inject(m, st, src.pts[q], it, c->stream);
+        launches += 2 + (int64_t)src.vfall.size() + (int64_t)src.sfall.size();
+        if (save) boundary_strips(m, st, *save, it, true, c->stream, &launches);
+        smp.done(it);
+    }
+    if (save)   // boundary_save_snap_tau!: -stress at nt (boundary.jl:268-287)
+        for (int q = 0; q < save->nfield; ++q) {
+            negate_copy(save->snap[save->field[q]], st.field[save->field[q]], m.slab, c->stream);
+            ++launches;
+        }
+    if ((rc = velocity_pass(c, m, st, X0))) return rc;   // :254-255
+    ++launches;
+    if (save)   // boundary_save_snap_v!: v at nt+1/2 (boundary.jl:289-306)
+        for (int f : {GPI_VX, GPI_VY, GPI_VZ})
+            if (save->snap[f])
+                HIPTRY(c, hipMemcpyAsync(save->snap[f], st.field[f], sizeof(float) * (size_t)m.slab,
+                                         hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = finish_timing(c, smp, nt, launches))) return rc;
+
+    // copy back
+    for (int q = 0; q < shot->nrec; ++q) {
+        const DevPoints& D = rec[q];
+        std::vector<float> buf((size_t)nt * std::max(D.npts, 1));
+        HIPTRY(c, hipMemcpy(buf.data(), D.rec, sizeof(float) * buf.size(), hipMemcpyDeviceToHost));
+        float* o = shot->records[q];
+        for (int r = 0; r < D.npts; ++r)
+            for (int64_t it = 0; it < nt; ++it) o[(size_t)r * nt + it] = buf[(size_t)it * D.npts + r];
+    }
+    for (int q = 0; q < shot->nfinal; ++q) {
+        const int f = shot->final_fields[q];
+        int64_t n3[3];
+        shape_of(g.ndim, g.order, g.physics, g.n, f, n3);
+        if ((rc = download_pitched(c, m, shot->final_out[q], st.field[f], n3))) return rc;
+    }
+    return GPI_OK;
+}
+
+int check_shot(gpi_ctx* c, uint64_t model, const gpi_shot* shot, Model** out)
+{
+    if (!c || !shot) return fail(c, GPI_EINVAL, "null argument");
+    Model* m = find_model(c, model);
+    if (!m) return fail(c, GPI_ENOTFOUND, "unknown model handle");
+    if (shot->nt < 1) return fail(c, GPI_EINVAL, "nt must be positive");
+    if (shot->nsrc < 0 || shot->nrec < 0 || shot->nsrc > 8 || shot->nrec > 16 || shot->nfinal < 0 ||
+        (shot->nsrc > 0 && !shot->src) || (shot->nrec > 0 && !shot->rec) ||
+        (shot->nfinal > 0 && (!shot->final_fields || !shot->final_out)))
+        return fail(c, GPI_EINVAL, "bad number of source/receiver fields");
+    HIPTRY(c, hipSetDevice(c->device));
+    *out = m;
+    return GPI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
+{
+    Model* m = nullptr;
+    int rc = check_shot(c, model, shot, &m);
+    if (rc) return rc;
+    return run_shot(c, *m, shot, nullptr);
+}
+
+int gpi_forward_save(gpi_ctx* c, uint64_t model, const gpi_shot* shot, uint64_t* store_out)
+{
+    if (!store_out) return fail(c, GPI_EINVAL, "null argument");
+    *store_out = 0;
+    Model* m = nullptr;
+    int rc = check_shot(c, model, shot, &m);
+    if (rc) return rc;
+    Store* s = nullptr;
+    if ((rc = alloc_store(c, *m, shot->nt, &s))) return rc;
+    if ((rc = run_shot(c, *m, shot, s))) {
+        (void)hipStreamSynchronize(c->stream);
+        free_allocs(c, s->allocs, s->bytes);
+        delete s;
+        return rc;
+    }
+    c->stores.push_back(s);
+    *store_out = (uint64_t)(uintptr_t)s;
+    return GPI_OK;
+}
+
+int gpi_store_free(gpi_ctx* c, uint64_t store)
+{
+    if (!c) return GPI_EINVAL;
+    Store* s = find_store(c, store);
+    if (!s) return fail(c, GPI_ENOTFOUND, "unknown store handle");
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->stores.erase(std::find(c->stores.begin(), c->stores.end(), s));
+    free_allocs(c, s->allocs, s->bytes);
+    delete s;
+    return GPI_OK;
+}
+
+int gpi_adjoint(gpi_ctx* c, uint64_t model, uint64_t store, const gpi_adjoint_shot* shot, float dtI, float* g_invK,
+                float* g_rho)
+{
+    if (!c || !shot || !g_invK || !g_rho) return fail(c, GPI_EINVAL, "null argument");
+    Model* mp = find_model(c, model);
+    if (!mp) return fail(c, GPI_ENOTFOUND, "unknown model handle");
+    Store* sp = find_store(c, store);
+    if (!sp || sp->model != mp) return fail(c, GPI_ENOTFOUND, "unknown store handle for this model");
+    Model& m = *mp;
+    const Store& sv = *sp;
+    const gpi_grid& g = m.g;
+    if (g.physics != 0 || g.ndim != 2) return fail(c, GPI_EINVAL, "the reference gradient exists for 2-D acoustic only");
+    const int64_t nt = shot->nt;
+    if (nt != sv.nt) return fail(c, GPI_EINVAL, "nt differs from the stored forward run");
+    if (shot->nsrc1 < 0 || shot->nsrc2 < 0 || shot->nsrc1 > 8 || shot->nsrc2 > 8 ||
+        (shot->nsrc1 > 0 && !shot->src1) || (shot->nsrc2 > 0 && !shot->src2))
+        return fail(c, GPI_EINVAL, "bad number of source fields");
+    if (!(dtI > 0.0f) || !std::isfinite(dtI)) return fail(c, GPI_EINVAL, "dtI must be positive");
+    HIPTRY(c, hipSetDevice(c->device));
+    const State& s1 = *m.st0;
+    Scratch S(c);
+    // second wavefield: its own wavefields and CPML memory
+    struct Held {
+        gpi_ctx* c;
+        State st;
+        ~Held()
+        {
+            (void)hipStreamSynchronize(c->stream);
+            free_allocs(c, st.allocs, st.bytes);
+        }
+    } h2{c, State{}};
+    int rc = alloc_state(c, m, &h2.st);
+    if (rc) return rc;
+    const State& s2 = h2.st;
+    PassExtras XV1{}, XS1{}, XV2{}, XS2{};
+    Sources src1, src2;
+    if ((rc = plan_sources(c, m, S, shot->nsrc1, shot->src1, shot->wav1, nt, false, XV1, XS1, src1))) return rc;
+    if ((rc = plan_sources(c, m, S, shot->nsrc2, shot->src2, shot->wav2, nt, true, XV2, XS2, src2))) return rc;
+    // previous-step copies (save_tp!) in two alternating sets, and the gradients
+    float* tp[2][6];
+    for (auto& set : tp)
+        for (float*& p : set)
+            if ((rc = S.alloc(&p, (size_t)m.slab))) return rc;
+    float *gK, *gR;
+    if ((rc = S.alloc(&gK, (size_t)m.slab)) || (rc = S.alloc(&gR, (size_t)m.slab))) return rc;
+    const size_t bytes = sizeof(float) * (size_t)m.slab;
+    HIPTRY(c, hipMemsetAsync(gK, 0, bytes, c->stream));
+    HIPTRY(c, hipMemsetAsync(gR, 0, bytes, c->stream));
+
+    // initial conditions: the first wavefield from the snapshots (boundary_force_snap_tau!,
+    // boundary_force_snap_v!, boundary.jl:173-212), the second at rest
+    if ((rc = zero_state(c, m, s1, c->stream)) || (rc = zero_state(c, m, s2, c->stream))) return rc;
+    for (int f : {GPI_P, GPI_VX, GPI_VZ})
+        HIPTRY(c, hipMemcpyAsync(s1.field[f], sv.snap[f], bytes, hipMemcpyDeviceToDevice, c->stream));
+    const float* cur[6] = {s1.field[GPI_P], s1.field[GPI_VX], s1.field[GPI_VZ],
+                           s2.field[GPI_P], s2.field[GPI_VX], s2.field[GPI_VZ]};
+    for (int k = 0; k < 6; ++k) {
+        if (k < 3) HIPTRY(c, hipMemcpyAsync(tp[0][k], cur[k], bytes, hipMemcpyDeviceToDevice, c->stream));
+        else HIPTRY(c, hipMemsetAsync(tp[0][k], 0, bytes, c->stream));
+    }
+
+    // time loop of :adjoint with npw = 2 (propagate.jl:170-247): save_tp!, boundary_force!
+    // on the first wavefield, both wavefields' passes and sources, then compute_gradient!
+    Sampler smp(c, nt);
+    int64_t launches = 0;
+    HIPTRY(c, hipEventRecord(c->ev[0], c->stream));
+    for (int64_t it = 0; it < nt; ++it) {
+        hipEvent_t* E = smp.at(it);
+        XV1.it = XS1.it = XV2.it = XS2.it = it;
+        boundary_strips(m, s1, sv, nt - 1 - it, false, c->stream, &launches);
+        if (E) HIPTRY(c, hipEventRecord(E[0], c->stream));
+        if ((rc = velocity_pass(c, m, s1, XV1)) || (rc = velocity_pass(c, m, s2, XV2))) return rc;
+        if (E) HIPTRY(c, hipEventRecord(E[1], c->stream));
+        for (int q : src1.vfall) inject(m, s1, src1.pts[q], it, c->stream);
+        for (int q : src2.vfall) inject(m, s2, src2.pts[q], it, c->stream);
+        if (E) HIPTRY(c, hipEventRecord(E[2], c->stream));
+        if ((rc = stress_pass(c, m, s1, XS1)) || (rc = stress_pass(c, m, s2, XS2))) return rc;
+        if (E) HIPTRY(c, hipEventRecord(E[3], c->stream));
+        for (int q : src1.sfall) inject(m, s1, src1.pts[q], it, c->stream);
+        // gradlame! and gradrho! (gradient.jl:17-64), fused with the next step's save_tp!
+        image_acoustic_2d(m, cur, tp[it & 1], tp[(it + 1) & 1], gK, gR, dtI, c->stream);
+        launches += 5 + (int64_t)(src1.vfall.size() + src2.vfall.size() + src1.sfall.size());
+        smp.done(it);
+    }
+    if ((rc = finish_timing(c, smp, nt, launches))) return rc;
+    // the trailing velocity half step of :254-255 changes no output of this mode: omitted
+    int64_t n3[3];
+    shape_of(g.ndim, g.order, g.physics, g.n, GPI_P, n3);
+    if ((rc = download_pitched(c, m, g_invK, gK, n3)) || (rc = download_pitched(c, m, g_rho, gR, n3))) return rc;
+    return GPI_OK;
+}
+
+}  // extern "C"

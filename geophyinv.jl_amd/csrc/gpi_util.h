@@ -1,0 +1,100 @@
+// Host-side helpers shared by the C-ABI translation units: errors, field tables,
+// allocation accounting, pitched copies and per-call scratch buffers.
+#pragma once
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gpi_internal.h"
+
+struct gpi_ctx : public gpi::Ctx {};
+
+namespace gpi {
+
+inline int fail(gpi_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPTRY(ctx, expr)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::gpi::fail(ctx, GPI_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// Per-axis grid code of each field (src/fields.jl:92-671): N = n points, A = n+h,
+// H = n-h, I = n-2h.  '.' marks the absent y axis of 2-D grids.
+const char* field_codes(int ndim, int physics, int f);
+int shape_of(int ndim, int order, int physics, const int64_t n[3], int f, int64_t out[3]);
+// axis along which a derivative field differentiates (0 z, 1 y, 2 x); -1 for wavefields
+int deriv_axis(int f);
+// grid (field id) of each coefficient array (src/fdtd/medium.jl:81-95); -1 if unused
+int coef_grid(int ndim, int physics, int c);
+bool required_coef(int ndim, int physics, int c);
+
+inline bool is_wavefield(int f) { return f >= GPI_P && f <= GPI_TAUYZ; }
+inline bool is_velocity(int f) { return f == GPI_VX || f == GPI_VY || f == GPI_VZ; }
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+Model* find_model(gpi_ctx* c, uint64_t h);
+int dev_alloc(gpi_ctx* c, std::vector<void*>* owner, uint64_t* owner_bytes, void** p, size_t bytes);
+void free_allocs(gpi_ctx* c, std::vector<void*>& allocs, uint64_t& bytes);
+void free_model(gpi_ctx* c, Model* m);
+
+// wavefields + CPML memory of one propagating wavefield of model m
+int alloc_state(gpi_ctx* c, const Model& m, State* st);
+void free_state(gpi_ctx* c, State* st);
+int zero_state(gpi_ctx* c, const Model& m, const State& st, hipStream_t s);
+size_t psi_floats(const Model& m, int f);
+
+// padded offset of compact index `lin` of an array of logical size n[3]
+inline int64_t padded(const Model& m, const int64_t n[3], int64_t lin)
+{
+    const int64_t iz = lin % n[0];
+    const int64_t r = lin / n[0];
+    const int64_t iy = r % n[1];
+    const int64_t ix = r / n[1];
+    return iz + m.pitch0 * (iy + m.pitch1 * ix);
+}
+
+int upload_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3]);
+int download_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3]);
+
+// Scratch device buffers of one call, freed on every exit path (after the stream drained).
+struct Scratch {
+    gpi_ctx* c;
+    std::vector<void*> p;
+    uint64_t bytes = 0;
+    explicit Scratch(gpi_ctx* cc) : c(cc) {}
+    ~Scratch()
+    {
+        (void)hipStreamSynchronize(c->stream);
+        free_allocs(c, p, bytes);
+    }
+    template <typename T>
+    int alloc(T** out, size_t count)
+    {
+        void* q = nullptr;
+        const size_t b = std::max<size_t>(sizeof(T) * count, 4);
+        int rc = dev_alloc(c, &p, &bytes, &q, b);
+        if (rc) return rc;
+        *out = (T*)q;
+        return GPI_OK;
+    }
+    template <typename T>
+    int put(T** out, const std::vector<T>& v)
+    {
+        int rc = alloc(out, v.size());
+        if (rc) return rc;
+        if (!v.empty()) {
+            hipError_t e = hipMemcpy(*out, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+            if (e != hipSuccess) return fail(c, GPI_EHIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+        }
+        return GPI_OK;
+    }
+};
+
+}  // namespace gpi

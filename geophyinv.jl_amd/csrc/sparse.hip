@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void inject_kernel(InjectParams P)
 
 }  // namespace
 
-void inject(const Model& m, const DevPoints& pts, int64_t it, hipStream_t s)
+void inject(const Model& m, const State& st, const DevPoints& pts, int64_t it, hipStream_t s)
 {
     if (pts.ntarget == 0) return;
     InjectParams P{};
@@ -79,18 +79,18 @@ void inject(const Model& m, const DevPoints& pts, int64_t it, hipStream_t s)
     const int f = pts.field;
     if (f == GPI_VX || f == GPI_VY || f == GPI_VZ) {
         P.kind = INJ_VEL;
-        P.f0 = m.field[f];
+        P.f0 = st.field[f];
         P.rho = m.coef[GPI_C_RHO];
         P.roff = pts.toff + pts.ntarget;   // rho offsets stored after the target offsets
     } else if (m.g.physics == 0) {
         P.kind = INJ_P;
-        P.f0 = m.field[GPI_P];
+        P.f0 = st.field[GPI_P];
         P.coef = m.coef[GPI_C_DTK];
     } else {
         P.kind = m.g.ndim == 3 ? INJ_NORMAL3 : INJ_NORMAL2;
-        P.f0 = m.field[GPI_TAUXX];
-        P.f1 = m.field[GPI_TAUZZ];
-        P.f2 = m.g.ndim == 3 ? m.field[GPI_TAUYY] : nullptr;
+        P.f0 = st.field[GPI_TAUXX];
+        P.f1 = st.field[GPI_TAUZZ];
+        P.f2 = m.g.ndim == 3 ? st.field[GPI_TAUYY] : nullptr;
         P.coef = m.coef[GPI_C_DTM];
     }
     hipLaunchKernelGGL(inject_kernel, dim3((pts.ntarget + 255) / 256), dim3(256), 0, s, P);

@@ -4,7 +4,9 @@ The names mirror the reference's Julia API: ``SeisForwExpt``, ``update`` (``upda
 ``AcousticMedium``/``ElasticMedium``, ``AGeomss``/``AGeom``, ``Srcs``/``Recs``
 (``SrcWav``/``Records``), ``ricker``, ``FdtdAcoustic``/``FdtdElastic``.
 """
-from .database import AGeomss, Records, Recs, SrcWav, Srcs, gradient, lossvalue
+from . import database as _database
+from . import func_grad as _func_grad
+from .database import AGeomss, L2DistLoss, Records, Recs, SrcWav, Srcs
 from .fdtd import FdtdAcoustic, FdtdElastic, SeisForwExpt, update
 from .fields import FIELD_ID, FIELDS, field_shape, get_mgrid
 from .grid import Grid
@@ -13,8 +15,25 @@ from .wavelets import findfreq, ricker
 
 AGeom = list
 
+
+def lossvalue(*args, **kw):
+    """``lossvalue(loss, dobs, dcal)`` (database.jl:357-368) or
+    ``lossvalue(m, loss, dobs, pa[, mparams])`` (func_grad.jl:1-9)."""
+    if len(args) >= 4:
+        return _func_grad.lossvalue(*args, **kw)
+    return _database.lossvalue(*args, **kw)
+
+
+def gradient(*args, **kw):
+    """``gradient!(buffer, loss, dobs, dcal)`` (database.jl:371-380) or
+    ``gradient!(g, m, loss, dobs, pa[, mparams])`` (func_grad.jl:11-49)."""
+    if len(args) >= 5:
+        return _func_grad.gradient(*args, **kw)
+    return _database.gradient(*args, **kw)
+
+
 __all__ = [
     "AGeom", "AGeomss", "AcousticMedium", "ElasticMedium", "FIELDS", "FIELD_ID", "FdtdAcoustic",
-    "FdtdElastic", "Grid", "Medium", "Records", "Recs", "SeisForwExpt", "SrcWav", "Srcs",
+    "FdtdElastic", "Grid", "L2DistLoss", "Medium", "Records", "Recs", "SeisForwExpt", "SrcWav", "Srcs",
     "field_shape", "findfreq", "get_mgrid", "gradient", "lossvalue", "padarray", "ricker", "update",
 ]

@@ -54,6 +54,12 @@ class gpi_shot(C.Structure):
                 ("final_out", C.POINTER(C.POINTER(C.c_float)))]
 
 
+class gpi_adjoint_shot(C.Structure):
+    _fields_ = [("nt", C.c_int64), ("nsrc1", C.c_int32), ("nsrc2", C.c_int32),
+                ("src1", C.POINTER(gpi_points)), ("wav1", C.POINTER(C.POINTER(C.c_float))),
+                ("src2", C.POINTER(gpi_points)), ("wav2", C.POINTER(C.POINTER(C.c_float)))]
+
+
 class gpi_timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("velocity_avg_ms", C.c_double), ("stress_avg_ms", C.c_double),
                 ("steps", C.c_int64), ("sampled", C.c_int64), ("launches", C.c_int64)]
@@ -72,6 +78,10 @@ SIGNATURES = {
     "gpi_model_update": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gpi_array), C.c_int32]),
     "gpi_model_free": (C.c_int, [C.c_void_p, C.c_uint64]),
     "gpi_forward": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gpi_shot)]),
+    "gpi_forward_save": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gpi_shot), C.POINTER(C.c_uint64)]),
+    "gpi_adjoint": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(gpi_adjoint_shot), C.c_float,
+                              C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "gpi_store_free": (C.c_int, [C.c_void_p, C.c_uint64]),
     "gpi_last_timing": (C.c_int, [C.c_void_p, C.POINTER(gpi_timing)]),
     "gpi_set_pass_sampling": (C.c_int, [C.c_void_p, C.c_int32]),
 }

@@ -109,21 +109,46 @@ class AGeomss:
         return True
 
 
-def lossvalue(dobs, dcal) -> float:
-    """Sum over supersources, fields, samples of the squared difference (L2DistLoss), FP64."""
+class L2DistLoss:
+    """``LossFunctions.L2DistLoss`` (LossFunctions 0.11.2): value (output - target)^2 and
+    derivative 2 (output - target)."""
+
+    def value(self, output, target):
+        return (output - target) ** 2
+
+    def deriv(self, output, target):
+        return 2.0 * (output - target)
+
+
+def _loss(loss):
+    if loss is None or isinstance(loss, L2DistLoss) or loss == "L2DistLoss":
+        return L2DistLoss()
+    raise NotImplementedError("only L2DistLoss is available")
+
+
+def lossvalue(loss, dobs, dcal) -> float:
+    """``lossvalue(loss, dobs, dcal)`` (src/database/database.jl:357-368): the sum over
+    supersources, fields, samples and receivers of ``loss(dobs, dcal)``, accumulated in FP64."""
+    L = _loss(loss)
     if isinstance(dobs, list):
-        return float(sum(lossvalue(a, b) for a, b in zip(dobs, dcal)))
-    return float(sum(np.sum((dobs.d[f].astype(np.float64) - dcal.d[f].astype(np.float64)) ** 2)
+        return float(sum(lossvalue(L, a, b) for a, b in zip(dobs, dcal)))
+    return float(sum(np.sum(L.value(dobs.d[f].astype(np.float64), dcal.d[f].astype(np.float64)))
                      for f in dobs.fields))
 
 
-def gradient(buffer, dobs, dcal):
-    """Adjoint source ``deriv(L2DistLoss(), dobs, dcal)`` into ``buffer`` (FP64 then FP32)."""
+def gradient(buffer, loss, dobs, dcal):
+    """``gradient!(buffer, loss, dobs, dcal)`` (src/database/database.jl:371-380): the adjoint
+    source ``deriv(loss, dobs, dcal)`` of every sample into ``buffer`` (FP32).
+
+    LossFunctions 0.11 orders the arguments as (output, target), so the call gives
+    2 (dobs - dcal).  With that sign the reference's time-reversal gradient agrees with
+    finite differences of the misfit."""
+    L = _loss(loss)
     if isinstance(dobs, list):
         for g, a, b in zip(buffer, dobs, dcal):
-            gradient(g, a, b)
+            gradient(g, L, a, b)
         return buffer
     for f in dobs.fields:
-        # LossFunctions 0.11 deriv(L2DistLoss(), target, output) = 2*(output - target)
-        buffer.d[f][:, :] = (2.0 * (dcal.d[f].astype(np.float64) - dobs.d[f].astype(np.float64))).astype(np.float32)
+        buffer.d[f][:, :] = L.deriv(dobs.d[f].astype(np.float64), dcal.d[f].astype(np.float64)).astype(np.float32)
     return buffer
+

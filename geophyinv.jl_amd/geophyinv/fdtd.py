@@ -18,11 +18,12 @@ import numpy as np
 
 from . import _lib
 from .cpml import pml_vectors
-from .database import Recs
+from .database import Recs, Srcs
 from .fields import (COEF_ID, FIELD_ID, coef_grid, derivative_fields, field_shape, get_mgrid,
                      has_field, wavefields_of)
 from .grid import Grid
-from .media import AcousticMedium, ElasticMedium, Medium, coefficients, padarray
+from .media import (MPARAMS, AcousticMedium, ElasticMedium, Medium, MediumParameter, dependent, independent,
+                    padarray)
 from .projmat import get_proj_matrix
 from .wavelets import findfreq
 
@@ -112,8 +113,8 @@ class SeisForwExpt:
         ndim = medium.ndim
         if order not in (2, 4, 6, 8):
             raise ValueError("order must be 2, 4, 6 or 8")
-        if attrib_mod.mode != "forward" or attrib_mod.npw != 1:
-            raise NotImplementedError("only the :forward mode with npw = 1 is available")
+        if attrib_mod.npw not in (1, 2) or (attrib_mod.mode == "adjoint" and attrib_mod.npw != 2):
+            raise ValueError("npw must be 1 or 2, and the :adjoint mode needs npw = 2")
         if isinstance(medium, AcousticMedium) != isinstance(attrib_mod, FdtdAcoustic):
             raise ValueError("medium type does not match the modelling attribute")
         physics = attrib_mod.physics
@@ -163,20 +164,30 @@ class SeisForwExpt:
         div = {2: 1.0, 4: 24.0, 6: 1920.0, 8: 107520.0}[order]
         self.dt32 = F32(tgrid.step)
         self.dsI = [F32(1.0 / (g.step * div)) for g in self.exmedium.grid]
-        self.coefs = coefficients(self.exmedium, self.dt32, order)
-        # ---- wavelets and frequency bounds (src/fdtd/source.jl:24-58, 192-246)
-        self.src_types = [1]
+        # ---- pac.mod, its reference values and pac.dmod (src/fdtd/fdtd.jl:170-176, medium.jl:131-186)
+        self.mparams = list(MPARAMS["acoustic" if physics == 0 else "elastic"])
+        self.mod = independent(self.exmedium)
+        self.ref_mod = {k: MediumParameter(v).ref for k, v in self.mod.items()}
+        self.coefs = dependent(self.mod, self.dt32, order, ndim)
+        # ---- wavelets and frequency bounds (src/fdtd/source.jl:24-58, 192-246); with npw = 2
+        # the second wavefield's sources sit at the receivers (fdtd.jl:90-95)
+        self.npw = attrib_mod.npw
+        self.srcwav2 = [Srcs(a.nr, tgrid, rfields) for a in ageom] if self.npw == 2 else []
+        self.src_types = [1] * self.npw
         self._fill_wavelets()
         # ---- CPML (src/fdtd/cpml.jl:144-155): velavg = mean(exmedium[:vp].bounds)
         b = self.exmedium.vp.bounds
         self.velavg = F32(F32(b[0] + b[1]) / F32(2))
         self.pml = pml_vectors(physics, self.exmedium.grid, self.medium.grid, self.pml_faces,
                                order, self.npml, self.dt32, self.velavg, self.freqpeak)
+        self.pml_freqpeak = self.freqpeak   # the CPML profile keeps the construction-time peak
         check_stability(self.medium, float(self.dt32), self.freqmax, order, verbose)
         self._update_weights()
         self.data = [Recs(a.nr, tgrid, rfields) for a in ageom]
+        self.gradients = {k: np.zeros(self.ext_shape, F32, order="F") for k in self.mparams}
         self._ctx = context
         self._model = None
+        self._stores = {}
         self.timing = None
 
     # ------------------------------------------------------------------ host preparation
@@ -212,6 +223,18 @@ class SeisForwExpt:
         self.freqmin = F32(max(fmins) if fmins else 0.0)
         self.freqmax = F32(min(fmaxs) if fmaxs else math.inf)
         self.freqpeak = F32(np.mean(fpeaks) if fpeaks else 0.0)
+        # second wavefield: velocity sources only are ever injected (source.jl:141-156)
+        self.wavelets2 = []
+        for s in self.srcwav2:
+            per = {}
+            for f in s.fields:
+                if f not in ("vx", "vy", "vz"):
+                    continue
+                full = np.zeros((nt, s.n), F32, order="F")
+                snt = min(len(s.grid), nt)
+                full[:snt] = get_source(s.d[f], f, self.src_types[1])[:snt]
+                per[f] = full
+            self.wavelets2.append(per)
 
     def _update_weights(self):
         grid = self.exmedium.grid
@@ -282,27 +305,43 @@ class SeisForwExpt:
 
     def free(self):
         if self._model is not None and self._ctx is not None and self._ctx.h:
-            self._ctx.check(self._ctx.lib.gpi_model_free(self._ctx.h, self._model))
+            self._ctx.check(self._ctx.lib.gpi_model_free(self._ctx.h, self._model))   # frees its stores
         self._model = None
+        self._stores = {}
 
-    def run_shot(self, iss: int, final_fields=()):
-        """Propagate supersource ``iss`` on the GPU; returns (records dict, finals dict)."""
-        self._ensure_model()
-        ctx = self._ctx
-        nt = len(self.tgrid)
-        keep = []
-        srcs = []
-        wavs = []
-        for f, (idx, w) in self.spray[iss].items():
+    def _free_store(self, iss):
+        h = self._stores.pop(iss, None)
+        if h is not None and self._ctx is not None and self._ctx.h:
+            self._ctx.check(self._ctx.lib.gpi_store_free(self._ctx.h, h))
+
+    @staticmethod
+    def _points(spray: dict, wavelets: dict, keep: list):
+        pts, wavs = [], []
+        for f, (idx, w) in spray.items():
+            if f not in wavelets:
+                continue
             idx = np.ascontiguousarray(idx)
             w = np.ascontiguousarray(w)
-            wv = np.asfortranarray(self.wavelets[iss][f], dtype=F32)
+            wv = np.asfortranarray(wavelets[f], dtype=F32)
             keep += [idx, w, wv]
             p = _lib.gpi_points()
             p.field, p.npts, p.nnz = FIELD_ID[f], idx.shape[0], idx.shape[1]
             p.idx, p.w = _lib.iptr(idx), _lib.fptr(w)
-            srcs.append(p)
+            pts.append(p)
             wavs.append(_lib.fptr(wv))
+        return ((_lib.gpi_points * max(1, len(pts)))(*pts), (C.POINTER(C.c_float) * max(1, len(wavs)))(*wavs),
+                len(pts))
+
+    def run_shot(self, iss: int, final_fields=(), save: bool = False):
+        """Propagate supersource ``iss`` on the GPU; returns (records dict, finals dict).
+
+        ``save`` runs the :forward_save variant, which keeps the boundary strips and the
+        final snapshots of this supersource resident for a later :adjoint run."""
+        self._ensure_model()
+        ctx = self._ctx
+        nt = len(self.tgrid)
+        keep = []
+        src_arr, wav_arr, nsrc = self._points(self.spray[iss], self.wavelets[iss], keep)
         recs, outs, names = [], [], []
         for f, (idx, w) in self.interp[iss].items():
             idx = np.ascontiguousarray(idx)
@@ -323,27 +362,115 @@ class SeisForwExpt:
             fptrs.append(_lib.fptr(a))
         shot = _lib.gpi_shot()
         shot.nt = nt
-        shot.nsrc = len(srcs)
+        shot.nsrc = nsrc
         shot.nrec = len(recs)
-        shot.src = (_lib.gpi_points * max(1, len(srcs)))(*srcs)
-        shot.wavelets = (C.POINTER(C.c_float) * max(1, len(wavs)))(*wavs)
+        shot.src = src_arr
+        shot.wavelets = wav_arr
         shot.rec = (_lib.gpi_points * max(1, len(recs)))(*recs)
         shot.records = (C.POINTER(C.c_float) * max(1, len(outs)))(*[_lib.fptr(o) for o in outs])
         fids = np.array([FIELD_ID[f] for f in final_fields], np.int32)
         shot.nfinal = len(final_fields)
         shot.final_fields = fids.ctypes.data_as(C.POINTER(C.c_int32))
         shot.final_out = (C.POINTER(C.c_float) * max(1, len(fptrs)))(*fptrs)
-        ctx.check(ctx.lib.gpi_forward(ctx.h, self._model, C.byref(shot)))
+        if save:
+            self._free_store(iss)
+            h = C.c_uint64()
+            ctx.check(ctx.lib.gpi_forward_save(ctx.h, self._model, C.byref(shot), C.byref(h)))
+            self._stores[iss] = h.value
+        else:
+            ctx.check(ctx.lib.gpi_forward(ctx.h, self._model, C.byref(shot)))
         self.timing = ctx.timing()
         return dict(zip(names, outs)), finals
 
+    def run_adjoint(self, iss: int):
+        """:adjoint run of supersource ``iss`` (npw = 2, 2-D acoustic): returns its
+        gradients (``pap[1].ss[iss].gradients``) w.r.t. invK and rho on the padded grid."""
+        if self.physics != 0 or self.ndim != 2:
+            raise NotImplementedError("the reference gradient exists for 2-D acoustic only (gradient.jl:17-64)")
+        if iss not in self._stores:
+            raise RuntimeError("run the :forward_save mode before the :adjoint mode")
+        ctx = self._ctx
+        keep = []
+        s1, w1, n1 = self._points(self.spray[iss], self.wavelets[iss], keep)
+        # adjoint sources are sprayed with the first wavefield's receiver weights (source.jl:150)
+        s2, w2, n2 = self._points(self.interp[iss], self.wavelets2[iss], keep)
+        sh = _lib.gpi_adjoint_shot()
+        sh.nt = len(self.tgrid)
+        sh.nsrc1, sh.src1, sh.wav1 = n1, s1, w1
+        sh.nsrc2, sh.src2, sh.wav2 = n2, s2, w2
+        gk = np.zeros(self.ext_shape, F32, order="F")
+        gr = np.zeros(self.ext_shape, F32, order="F")
+        dtI = F32(1.0 / self.tgrid.step)   # fc[:dtI] (fdtd.jl:326)
+        ctx.check(ctx.lib.gpi_adjoint(ctx.h, self._model, self._stores[iss], C.byref(sh), float(dtI),
+                                      _lib.fptr(gk), _lib.fptr(gr)))
+        self.timing = ctx.timing()
+        return {"invK": gk, "rho": gr}
+
     def update(self):
-        """``update!(pa)``: model every supersource and fill ``pa["data"]``."""
+        """``update!(pa)`` in the experiment's mode (src/fdtd/propagate.jl:74-135).
+
+        :forward and :forward_save model every supersource and fill ``pa["data"]``
+        (:forward_save also keeps the boundary values); :adjoint leaves the data zeroed, as
+        ``initialize!(pa.c)`` does, and stacks the per-supersource gradients into
+        ``pa.gradients`` in supersource order (sum_grads!, gradient.jl:2-12)."""
+        mode = self.attrib_mod.mode
+        for d in self.data:
+            d.fill(0.0)
+        for g in self.gradients.values():
+            g.fill(0.0)
+        if mode == "adjoint":
+            self._ensure_model()
+            for iss in range(len(self.ageom)):
+                gi = self.run_adjoint(iss)
+                for k, g in gi.items():
+                    self.gradients[k][...] = self.gradients[k] + g
+            return self.timing
         for iss in range(len(self.ageom)):
-            recs, _ = self.run_shot(iss)
+            recs, _ = self.run_shot(iss, save=(mode == "forward_save"))
             for f, r in recs.items():
                 self.data[iss].d[f][:, :] = r
         return self.timing
+
+    # ------------------------------------------------------------------ model vectors
+    def view_inner(self, a: np.ndarray) -> tuple:
+        """Slices of the unpadded region (``view_inner``, src/fdtd/medium.jl:13-26)."""
+        axes = ["z", "x"] if self.ndim == 2 else ["z", "y", "x"]
+        out = []
+        for ax, n in zip(axes, a.shape):
+            lo = self.npml if f"{ax}min" in self.pml_faces else 0
+            hi = n - self.npml if f"{ax}max" in self.pml_faces else n
+            out.append(slice(lo, hi))
+        return tuple(out)
+
+    def _upload_coefs(self):
+        self.coefs = dependent(self.mod, self.dt32, self.order, self.ndim)
+        if self._model is not None:
+            keep, arrs = self.coef_arrays()
+            self._ctx.check(self._ctx.lib.gpi_model_update(self._ctx.h, self._model, arrs, len(arrs)))
+
+    def update_model(self, m: np.ndarray, mparams=None):
+        """``update!(pa, m, mparams)`` (src/fdtd/medium.jl:31-52): the inner part of each
+        parameter becomes ``exp(m) * ref``; the padded part keeps its values."""
+        mparams = list(mparams or self.mparams)
+        chunks = np.split(np.asarray(m, dtype=np.float64), len(mparams))
+        for x, name in zip(chunks, mparams):
+            sl = self.view_inner(self.mod[name])
+            shape = self.mod[name][sl].shape
+            x32 = x.reshape(shape, order="F").astype(F32)
+            self.mod[name][sl] = np.exp(x32) * self.ref_mod[name]
+        self._upload_coefs()
+
+    def model_vector(self, mparams=None) -> np.ndarray:
+        """``update!(m, pa, mparams)`` (src/fdtd/medium.jl:55-76): ``log(mod / ref)`` of the
+        inner parts, concatenated; ``mod`` goes through the same FP32 round trip."""
+        mparams = list(mparams or self.mparams)
+        out = []
+        for name in mparams:
+            r = self.ref_mod[name]
+            lg = np.log(self.mod[name] * (F32(1) / r))
+            out.append(np.asarray(lg[self.view_inner(lg)], np.float64).ravel(order="F"))
+            self.mod[name][...] = np.exp(lg) * r
+        return np.concatenate(out)
 
     def update_medium(self, medium: Medium):
         """``update!(pa, medium)`` without reallocation (src/fdtd/medium.jl:128-140)."""
@@ -351,19 +478,25 @@ class SeisForwExpt:
             raise ValueError("medium must match the experiment's medium type and grid")
         self.medium = medium.copy()
         self.exmedium = padarray(self.medium, self.npml, self.pml_faces)
-        self.coefs = coefficients(self.exmedium, self.dt32, self.order)
-        if self._model is not None:
-            keep, arrs = self.coef_arrays()
-            self._ctx.check(self._ctx.lib.gpi_model_update(self._ctx.h, self._model, arrs, len(arrs)))
+        self.mod = independent(self.exmedium)
+        self._upload_coefs()
 
     def update_srcwav(self, srcwav, src_types=None):
-        """``update!(pa, srcwav, src_types)`` (src/fdtd/source.jl:192-246)."""
+        """``update!(pa, srcwav, src_types)`` (src/fdtd/source.jl:192-246).
+
+        ``srcwav`` is a SrcWav (one Srcs per supersource) for the first wavefield, or, with
+        npw = 2, a list [SrcWav, SrcWav] for both; ``src_types`` has one entry per wavefield."""
         srcwav = list(srcwav) if isinstance(srcwav, (list, tuple)) else [srcwav]
-        if len(srcwav) != len(self.srcwav):
-            raise ValueError("srcwav must have one entry per supersource")
-        self.srcwav = [s.copy() for s in srcwav]
+        per_pw = bool(srcwav) and isinstance(srcwav[0], (list, tuple))
+        sw = [list(x) for x in srcwav] if per_pw else [srcwav]
+        if len(sw) > self.npw or any(len(x) != len(self.srcwav) for x in sw):
+            raise ValueError("srcwav must have one entry per supersource (and per wavefield)")
+        self.srcwav = [s.copy() for s in sw[0]]
+        if len(sw) == 2:
+            self.srcwav2 = [s.copy() for s in sw[1]]
         if src_types is not None:
-            self.src_types = list(src_types) if isinstance(src_types, (list, tuple)) else [src_types]
+            st = list(src_types) if isinstance(src_types, (list, tuple)) else [src_types]
+            self.src_types[:len(st)] = st
         self._fill_wavelets()
         self._update_weights()
 
@@ -386,18 +519,26 @@ class SeisForwExpt:
             iss = None
         if key == "data":
             return self.data if iss is None else self.data[iss]
-        if key in ("ageom", "srcwav", "medium", "exmedium", "rfields", "pml_faces", "rigid_faces"):
+        if key in ("ageom", "srcwav", "medium", "exmedium", "rfields", "pml_faces", "rigid_faces", "gradients",
+                   "mod", "ref_mod"):
             return getattr(self, key)
         raise KeyError(key)
 
 
 def update(obj, *args, **kw):
-    """Dispatch of the reference's ``update!`` methods on a ``SeisForwExpt``."""
+    """Dispatch of the reference's ``update!`` methods on a ``SeisForwExpt``;
+    ``update(m, pa, mparams)`` fills the model vector ``m`` from ``pa``."""
+    if isinstance(obj, np.ndarray) and args and isinstance(args[0], SeisForwExpt):
+        m = args[0].model_vector(*args[1:])
+        obj[...] = m
+        return obj
     if not isinstance(obj, SeisForwExpt):
         raise TypeError("update expects a SeisForwExpt")
     if not args:
         return obj.update()
     a = args[0]
+    if isinstance(a, np.ndarray):
+        return obj.update_model(a, *args[1:])
     if isinstance(a, Medium):
         return obj.update_medium(a)
     if isinstance(a, list) and a and hasattr(a[0], "ns") and hasattr(a[0], "s"):

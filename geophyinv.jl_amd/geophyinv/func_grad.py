@@ -1,0 +1,68 @@
+"""Misfit and gradient of a model vector (src/fdtd/func_grad.jl:1-49).
+
+``lossvalue(m, loss, dobs, pa, mparams)`` puts the log-parameter vector ``m`` into the
+experiment (``mod = exp(m) * ref`` on the unpadded region), models the data in the
+:forward_save mode and returns the misfit.  ``gradient(g, m, loss, dobs, pa, mparams)``
+also back-propagates the adjoint sources in the :adjoint mode and writes
+``g = g_mod * exp(m) * ref`` (the chain rule of func_grad.jl:33-45).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import database
+
+F32 = np.float32
+
+
+def _forward_save(pa):
+    mode_save = pa.attrib_mod.mode
+    pa.attrib_mod.mode = "forward_save"
+    pa.update_srcwav([pa.srcwav, pa.srcwav2], [1, 0])
+    pa.update()
+    return mode_save
+
+
+def lossvalue(m, loss, dobs, pa, mparams=None) -> float:
+    """``lossvalue(m, loss, dobs, pa, mparams)`` (func_grad.jl:1-9)."""
+    if pa.npw != 2:
+        raise ValueError("the experiment needs npw = 2")
+    pa.update_model(m, mparams)
+    mode_save = _forward_save(pa)
+    pa.attrib_mod.mode = mode_save
+    return database.lossvalue(loss, dobs, pa.data)
+
+
+def gradient(g, m, loss, dobs, pa, mparams=None) -> float:
+    """``gradient!(g, m, loss, dobs, pa, mparams)`` (func_grad.jl:11-49); fills ``g`` in place
+    and returns the misfit.
+
+    With ``pa.compat == 1`` the return value is the reference's: its :adjoint ``update!``
+    zeroes ``pa.c.data`` (``initialize!``, types.jl:169-176) before ``lossvalue`` reads it,
+    so it returns the misfit of zero data.  With ``compat == 0`` it returns the misfit of
+    the modelled data."""
+    if pa.npw != 2:
+        raise ValueError("the experiment needs npw = 2")
+    mparams = list(mparams or pa.mparams)
+    pa.update_model(m, mparams)
+    mode_save = _forward_save(pa)
+    J = database.lossvalue(loss, dobs, pa.data)
+    # adjoint sources, time reversed
+    database.gradient(pa.srcwav2, loss, dobs, pa.data)
+    for S in pa.srcwav2:
+        for f in S.fields:
+            S.d[f][:, :] = S.d[f][::-1].copy()
+    pa.update_srcwav([pa.srcwav, pa.srcwav2], [-1, 1])
+    pa.attrib_mod.mode = "adjoint"
+    pa.update()
+    x = np.split(np.asarray(m, np.float64), len(mparams))
+    out = []
+    for xi, name in zip(x, mparams):
+        gm = pa.gradients[name][pa.view_inner(pa.gradients[name])]
+        xr = xi.reshape(gm.shape, order="F")
+        # gm1 * exp(x1) * r in FP64, stored back into the FP32 gradient
+        gm32 = (gm.astype(np.float64) * np.exp(xr) * np.float64(pa.ref_mod[name])).astype(F32)
+        out.append(gm32.ravel(order="F"))
+    g[...] = np.concatenate(out)
+    pa.attrib_mod.mode = mode_save
+    return database.lossvalue(loss, dobs, pa.data) if pa.compat == 1 else J

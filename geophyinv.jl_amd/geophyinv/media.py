@@ -146,12 +146,26 @@ def _avg2(a, b):
     return (a + b) * F32(0.5)
 
 
+# independent parameters held per experiment (MediumParameters, src/fdtd/medium.jl:81-95)
+MPARAMS = {"acoustic": ["invK", "rho"], "elastic": ["invlambda", "invmu", "rho"]}
+
+
+def independent(exmedium: Medium) -> dict:
+    """``pac.mod``: the independent FP32 parameters of the padded medium."""
+    kind = "elastic" if isinstance(exmedium, ElasticMedium) else "acoustic"
+    return {k: (exmedium.rho.m if k == "rho" else derived(exmedium, k)) for k in MPARAMS[kind]}
+
+
 def coefficients(exmedium: Medium, dt32: np.float32, order: int) -> dict:
     """Return the coefficient arrays keyed by the reference's names (``mod`` and ``dmod``)."""
+    return dependent(independent(exmedium), dt32, order, exmedium.ndim)
+
+
+def dependent(mod: dict, dt32: np.float32, order: int, nd: int) -> dict:
+    """``update_dmod!`` (src/fdtd/medium.jl:143-186): ``mod`` plus the dt-scaled arrays."""
     h = order - 1
-    nd = exmedium.ndim
-    rho = exmedium.rho.m
-    out = {"rho": rho}
+    rho = mod["rho"]
+    out = dict(mod)
     if nd == 2:
         nz, nx = rho.shape
         out["dtinvavxirho"] = _f(dt32 / _avg2(rho[h:nz - h, 0:nx - h], rho[h:nz - h, 1:nx - h + 1]))
@@ -164,15 +178,11 @@ def coefficients(exmedium: Medium, dt32: np.float32, order: int) -> dict:
                                              rho[h:nz - h, 1:ny - h + 1, h:nx - h]))
         out["dtinvavzirho"] = _f(dt32 / _avg2(rho[0:nz - h, h:ny - h, h:nx - h],
                                              rho[1:nz - h + 1, h:ny - h, h:nx - h]))
-    if isinstance(exmedium, AcousticMedium):
-        invK = derived(exmedium, "invK")
-        out["invK"] = invK
-        out["dtK"] = _f((F32(1) / invK) * dt32)
+    if "invK" in mod:
+        out["dtK"] = _f((F32(1) / mod["invK"]) * dt32)
         return out
-    il = derived(exmedium, "invlambda")
-    im = derived(exmedium, "invmu")
-    out["invlambda"] = il
-    out["invmu"] = im
+    il = mod["invlambda"]
+    im = mod["invmu"]
     out["dtlambda"] = _f((F32(1) / il) * dt32)
     # inv(invλ) + 2.0*inv(invμ) is a host broadcast: the Float64 literal promotes
     m64 = (F32(1) / il).astype(np.float64) + 2.0 * (F32(1) / im).astype(np.float64)

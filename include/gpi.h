@@ -178,6 +178,38 @@ int gpi_model_free(gpi_ctx* ctx, uint64_t model);
  * final wavefields) to the host. */
 int gpi_forward(gpi_ctx* ctx, uint64_t model, const gpi_shot* shot);
 
+/* ---- time reversal and the reference gradient ---------------------------------------- */
+/* One supersource in :forward_save mode (src/fdtd/propagate.jl:138-261 with
+ * boundary_save! at :229 and the snapshots of :251-258, src/fdtd/boundary.jl:217-306):
+ * the forward run of gpi_forward that also keeps, on the device, -stress on the
+ * nbound = 3 planes just inside the PML of every face for every step, -stress at nt and
+ * the velocities at nt+1/2.  The store stays resident until gpi_store_free (or until its
+ * model is freed). */
+int gpi_forward_save(gpi_ctx* ctx, uint64_t model, const gpi_shot* shot, uint64_t* store_out);
+
+/* Inputs of the :adjoint run with npw = 2 (src/fdtd/fdtd.jl:90-99, source.jl:124-157). */
+typedef struct gpi_adjoint_shot {
+    int64_t nt;
+    int32_t nsrc1;                 /* first wavefield: the forward sources (velocity fields) */
+    int32_t nsrc2;                 /* second wavefield: adjoint sources at the receivers */
+    const gpi_points* src1;        /* the forward spray weights (ssprayw of pw1) */
+    const float* const* wav1;      /* their wavelets after get_source(type -1) (source.jl:8-19) */
+    const gpi_points* src2;        /* the receiver interpolation weights (rinterpolatew of pw1) */
+    const float* const* wav2;      /* time-reversed adjoint sources, nt x nr, time fastest */
+} gpi_adjoint_shot;
+
+/* The :adjoint run of one supersource for 2-D acoustic physics (the only case with a
+ * gradient in the reference, src/fdtd/gradient.jl:17-64): the first wavefield restarts
+ * from the store's snapshots and re-propagates backwards with its strips forced from the
+ * store (boundary.jl:115-212); the second propagates the adjoint sources; after each step
+ *   g_invK += p2_prev * (p1_prev - p1) * dtI
+ *   g_rho[i+h, j+h] -= @av_xi(vx2_prev (vx1 - vx1_prev) dtI) + @av_zi(vz2_prev (vz1 - vz1_prev) dtI)
+ * g_invK and g_rho are host outputs on the stress grid (FP32, per supersource, as the
+ * reference's pap.ss[i].gradients); dtI = Float32(1/dt) of the time grid. */
+int gpi_adjoint(gpi_ctx* ctx, uint64_t model, uint64_t store, const gpi_adjoint_shot* shot, float dtI,
+                float* g_invK, float* g_rho);
+int gpi_store_free(gpi_ctx* ctx, uint64_t store);
+
 /* Device timing of the last gpi_forward on this context. */
 int gpi_last_timing(const gpi_ctx* ctx, gpi_timing* out);
 

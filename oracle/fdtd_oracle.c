@@ -170,6 +170,13 @@ struct oracle {
     float* rec[O_MAXRF];    /* nt x nr per receiver field */
     float* buf;             /* spray buffer (largest field) */
     int64_t it_done;
+    /* time-reversal store of :forward_save (boundary.jl): per stored stress field and
+     * axis, nt strips of 2*nbound planes; snapshots of the stress and velocity fields */
+    int nstore;
+    int store_f[3];
+    arr_t store[3][3];      /* [field][axis]: dims of the field with the axis = 2*nbound, x nt */
+    arr_t snap[O_NFIELDS];
+    arr_t tp[O_NFIELDS];    /* save_tp! copies (adjoint) */
 };
 
 static int present(const oracle_t* S, int f)
@@ -925,6 +932,227 @@ oracle_t* oracle_create(const oracle_expt* e)
     return S;
 }
 
+/* ---------------------------------------------------------------------------------- */
+/* time reversal: boundary saving and forcing (src/fdtd/boundary.jl), gradient          */
+/* ---------------------------------------------------------------------------------- */
+
+#define NBOUND 3 /* _fd_nbound, src/GeoPhyInv.jl:92 */
+
+/* stress fields kept by boundary_save! (boundary.jl:217-264): p, or tauxx/tauxz/tauzz */
+static int stored_fields(const oracle_t* S, int out[3])
+{
+    if (S->e.physics == 0) { out[0] = O_P; return 1; }
+    out[0] = O_TXX; out[1] = O_TXZ; out[2] = O_TZZ;
+    return 3;
+}
+
+static int64_t strip_np(const oracle_t* S, int axis)
+{
+    /* boundary_{save,force}{x,y,z}! use the min face flag for both strips (boundary.jl:24,40) */
+    return ((S->e.pml_mask >> (2 * axis)) & 1) ? S->e.npml : 0;
+}
+
+/* boundary_half{x,y,z}!: store[.., a] <-> field[.., np + a] (a < nbound) and
+ * field[.., n - np - 2 nbound + a] (a >= nbound), all other indices full */
+static void strip_copy(const oracle_t* S, arr_t* F, float* st, int axis, int save)
+{
+    int64_t sd[3] = {F->n[0], F->n[1], F->n[2]};
+    sd[axis] = 2 * NBOUND;
+    const int64_t np = strip_np(S, axis), n = F->n[axis];
+    for (int64_t k = 0; k < sd[2]; ++k)
+        for (int64_t j = 0; j < sd[1]; ++j)
+            for (int64_t i = 0; i < sd[0]; ++i) {
+                int64_t a = axis == 0 ? i : axis == 1 ? j : k;
+                int64_t fi[3] = {i, j, k};
+                fi[axis] = a < NBOUND ? np + a : n - np - 2 * NBOUND + a;
+                int64_t so = i + sd[0] * (j + sd[1] * k);
+                int64_t fo = IX(F, fi[0], fi[1], fi[2]);
+                if (save) st[so] = F->a[fo];
+                else F->a[fo] = st[so];
+            }
+}
+
+static int64_t strip_size(const arr_t* F, int axis)
+{
+    int64_t sd[3] = {F->n[0], F->n[1], F->n[2]};
+    sd[axis] = 2 * NBOUND;
+    return sd[0] * sd[1] * sd[2];
+}
+
+static int store_alloc(oracle_t* S)
+{
+    if (S->nstore) return 0;
+    if (S->e.physics == 1 && S->e.ndim == 3) return bad("boundary saving of 3-D elastic is not in the reference");
+    S->nstore = stored_fields(S, S->store_f);
+    for (int q = 0; q < S->nstore; ++q)
+        for (int ax = 0; ax < 3; ++ax) {
+            if (S->e.ndim == 2 && ax == 1) continue;
+            arr_t* F = &S->f[S->store_f[q]];
+            arr_t* B = &S->store[q][ax];
+            B->n[0] = strip_size(F, ax); B->n[1] = S->e.nt; B->n[2] = 1;
+            B->a = calloc((size_t)(B->n[0] * B->n[1]), sizeof(float));
+        }
+    return 0;
+}
+
+/* boundary_save!(it): store -stress on the strips (boundary.jl:217-264) */
+static void boundary_save(oracle_t* S, int64_t it)
+{
+    for (int q = 0; q < S->nstore; ++q)
+        for (int ax = 2; ax >= 0; --ax) {   /* x, (y,) z as in the reference */
+            if (S->e.ndim == 2 && ax == 1) continue;
+            arr_t* F = &S->f[S->store_f[q]];
+            float* st = S->store[q][ax].a + S->store[q][ax].n[0] * it;
+            strip_copy(S, F, st, ax, 1);
+            for (int64_t m = 0; m < S->store[q][ax].n[0]; ++m) st[m] = st[m] * -1.0f;
+        }
+}
+
+/* boundary_force!(it): overwrite the strips with the store (boundary.jl:115-169) */
+static void boundary_force(oracle_t* S, int64_t it)
+{
+    for (int q = 0; q < S->nstore; ++q)
+        for (int ax = 2; ax >= 0; --ax) {
+            if (S->e.ndim == 2 && ax == 1) continue;
+            arr_t* F = &S->f[S->store_f[q]];
+            strip_copy(S, F, S->store[q][ax].a + S->store[q][ax].n[0] * it, ax, 0);
+        }
+}
+
+static void copy_field(arr_t* dst, const arr_t* src)
+{
+    if (!dst->a) {
+        dst->n[0] = src->n[0]; dst->n[1] = src->n[1]; dst->n[2] = src->n[2];
+        dst->a = malloc(sizeof(float) * (size_t)numel(src));
+    }
+    memcpy(dst->a, src->a, sizeof(float) * (size_t)numel(src));
+}
+
+static void reset_wavefields(oracle_t* S)
+{
+    for (int f = 0; f < O_NFIELDS; ++f) {
+        if (S->f[f].a) memset(S->f[f].a, 0, sizeof(float) * (size_t)numel(&S->f[f]));
+        if (S->mem[f].a) memset(S->mem[f].a, 0, sizeof(float) * (size_t)numel(&S->mem[f]));
+    }
+}
+
+static const int vel_fields[3] = {O_VX, O_VY, O_VZ};
+
+/* mod_x_proc! in :forward_save mode (propagate.jl:138-261 with boundary_save! at :229 and
+ * the final snapshots of :251-258) */
+int oracle_run_save(oracle_t* S)
+{
+    if (store_alloc(S)) return -1;
+    reset_wavefields(S);
+    for (int64_t it = 0; it < S->e.nt; ++it) {
+        record(S, it, 1);
+        velocity_half_step(S);
+        add_velocity_source(S, it);
+        record(S, it, 0);
+        stress_half_step(S);
+        add_stress_source(S, it);
+        boundary_save(S, it);
+    }
+    for (int q = 0; q < S->nstore; ++q) {   /* boundary_save_snap_tau!: -stress */
+        arr_t* D = &S->snap[S->store_f[q]];
+        copy_field(D, &S->f[S->store_f[q]]);
+        for (int64_t m = 0; m < numel(D); ++m) D->a[m] = D->a[m] * -1.0f;
+    }
+    velocity_half_step(S);
+    for (int v = 0; v < 3; ++v)             /* boundary_save_snap_v! */
+        if (S->f[vel_fields[v]].a) copy_field(&S->snap[vel_fields[v]], &S->f[vel_fields[v]]);
+    S->it_done = S->e.nt;
+    return 0;
+}
+
+/* save_tp!: copy the wavefields (no derivatives) before the step (save_tp.jl:5-11) */
+static void save_tp(oracle_t* S)
+{
+    for (int f = O_P; f <= O_TYZ; ++f)
+        if (S->f[f].a) copy_field(&S->tp[f], &S->f[f]);
+}
+
+/* mod_x_proc! in :adjoint mode with npw = 2 (propagate.jl:138-261): S1 re-propagates the
+ * forward wavefield backwards from its snapshots with the strips forced from the store and
+ * its (type -1) sources; S2 propagates the adjoint sources.  compute_gradient! after each
+ * step accumulates (gradient.jl:17-64, 2-D acoustic only):
+ *   g_invK += p2_prev * (p1_prev - p1) * dtI
+ *   g_rho[inn] -= @av_xi(vx2_prev (vx1 - vx1_prev) dtI) + @av_zi(vz2_prev (vz1 - vz1_prev) dtI) */
+int oracle_adjoint(oracle_t* S1, oracle_t* S2, float* g_invK, float* g_rho)
+{
+    if (S1->e.physics != 0 || S1->e.ndim != 2) return bad("the reference gradient exists for 2-D acoustic only");
+    if (!S1->nstore || !S1->snap[O_P].a) return bad("run oracle_run_save first");
+    const int h = S1->h;
+    const float dtI = (float)(1.0 / S1->e.dt);   /* fc[:dtI] = Data.Number(inv(dt)) */
+    reset_wavefields(S1);
+    reset_wavefields(S2);
+    copy_field(&S1->f[O_P], &S1->snap[O_P]);     /* boundary_force_snap_tau! */
+    copy_field(&S1->f[O_VX], &S1->snap[O_VX]);   /* boundary_force_snap_v! */
+    copy_field(&S1->f[O_VZ], &S1->snap[O_VZ]);
+    const arr_t* P = &S1->f[O_P];
+    const int64_t N = numel(P);
+    memset(g_invK, 0, sizeof(float) * (size_t)N);
+    memset(g_rho, 0, sizeof(float) * (size_t)N);
+    const int64_t nt = S1->e.nt;
+    for (int64_t it = 0; it < nt; ++it) {
+        record(S1, it, 1);            /* computed but not returned by the reference (rec_flags) */
+        save_tp(S1);
+        save_tp(S2);
+        boundary_force(S1, nt - 1 - it);
+        velocity_half_step(S1);
+        velocity_half_step(S2);
+        add_velocity_source(S1, it);
+        add_velocity_source(S2, it);
+        record(S1, it, 0);
+        stress_half_step(S1);
+        stress_half_step(S2);
+        add_stress_source(S1, it);
+        /* gradlame!: compute_gmodKI! */
+        const float *p1 = S1->f[O_P].a, *p1p = S1->tp[O_P].a, *p2p = S2->tp[O_P].a;
+        for (int64_t m = 0; m < N; ++m) g_invK[m] = g_invK[m] + (p2p[m] * (p1p[m] - p1[m])) * dtI;
+        /* gradrho!: compute_gmodrho! into buffers, then combine_gmodrho! */
+        const arr_t *VX = &S1->f[O_VX], *VZ = &S1->f[O_VZ];
+        const float *vx1 = VX->a, *vx1p = S1->tp[O_VX].a, *vx2p = S2->tp[O_VX].a;
+        const float *vz1 = VZ->a, *vz1p = S1->tp[O_VZ].a, *vz2p = S2->tp[O_VZ].a;
+        const int64_t nz = P->n[0], nx = P->n[2];
+        for (int64_t j = 0; j < nx - 2 * h; ++j)
+            for (int64_t i = 0; i < nz - 2 * h; ++i) {
+                int64_t a0 = IX(VX, i + h, 0, j), a1 = IX(VX, i + h, 0, j + 1);
+                int64_t b0 = IX(VZ, i, 0, j + h), b1 = IX(VZ, i + 1, 0, j + h);
+                float bx0 = (vx2p[a0] * (vx1[a0] - vx1p[a0])) * dtI;
+                float bx1 = (vx2p[a1] * (vx1[a1] - vx1p[a1])) * dtI;
+                float bz0 = (vz2p[b0] * (vz1[b0] - vz1p[b0])) * dtI;
+                float bz1 = (vz2p[b1] * (vz1[b1] - vz1p[b1])) * dtI;
+                int64_t g = IX(P, i + h, 0, j + h);
+                g_rho[g] = (g_rho[g] - (bx0 + bx1) * 0.5f) - (bz0 + bz1) * 0.5f;
+            }
+    }
+    velocity_half_step(S1);   /* :254-255 (pw1 only) */
+    return 0;
+}
+
+int oracle_set_wavelets(oracle_t* S, int q, const float* w, int64_t n)
+{
+    if (q < 0 || q >= S->e.nsf || n != S->e.nt * S->e.ns) return -1;
+    memcpy(S->wav[q], w, sizeof(float) * (size_t)n);
+    return 0;
+}
+
+int oracle_get_store(const oracle_t* S, int q, int axis, float* out, int64_t n)
+{
+    if (q < 0 || q >= S->nstore || !S->store[q][axis].a || n != S->store[q][axis].n[0] * S->store[q][axis].n[1])
+        return -1;
+    memcpy(out, S->store[q][axis].a, sizeof(float) * (size_t)n);
+    return 0;
+}
+
+int oracle_get_snap(const oracle_t* S, int f, float* out, int64_t n)
+{
+    if (f < 0 || f >= O_NFIELDS || !S->snap[f].a || numel(&S->snap[f]) != n) return -1;
+    memcpy(out, S->snap[f].a, sizeof(float) * (size_t)n);
+    return 0;
+}
+
 /* mod_x_proc! time loop for one supersource, forward mode: propagate.jl:138-261.
  * nsteps < 0 runs all nt steps plus the trailing velocity half step (:254-255). */
 int oracle_run(oracle_t* S, int64_t nsteps)
@@ -977,6 +1205,16 @@ int oracle_get_coef(const oracle_t* S, int c, float* out, int64_t n)
 {
     if (c < 0 || c >= O_NCOEF || !S->coef[c].a || numel(&S->coef[c]) != n) return -1;
     memcpy(out, S->coef[c].a, sizeof(float) * (size_t)n);
+    return 0;
+}
+
+/* test hook: overwrite a coefficient array (O_C_RHO is the padded density used by the
+ * velocity sources), for runs on coefficients prepared elsewhere */
+int oracle_set_coef(oracle_t* S, int c, const float* in, int64_t n)
+{
+    arr_t* A = c == O_C_RHO ? &S->rho : (c >= 0 && c < O_NCOEF ? &S->coef[c] : NULL);
+    if (!A || !A->a || numel(A) != n) return -1;
+    memcpy(A->a, in, sizeof(float) * (size_t)n);
     return 0;
 }
 
@@ -1044,5 +1282,8 @@ void oracle_destroy(oracle_t* S)
     free(S->vp.a); free(S->vs.a); free(S->rho.a); free(S->buf);
     for (int q = 0; q < O_MAXSF; ++q) { free(S->spray[q].idx); free(S->spray[q].w); free(S->wav[q]); }
     for (int q = 0; q < O_MAXRF; ++q) { free(S->gath[q].idx); free(S->gath[q].w); free(S->rec[q]); }
+    for (int q = 0; q < 3; ++q)
+        for (int a = 0; a < 3; ++a) free(S->store[q][a].a);
+    for (int f = 0; f < O_NFIELDS; ++f) { free(S->snap[f].a); free(S->tp[f].a); }
     free(S);
 }

@@ -59,12 +59,18 @@ typedef struct oracle oracle_t;
 int oracle_field_shape(int physics, int ndim, int order, const int64_t n[3], int f, int64_t out[3]);
 oracle_t* oracle_create(const oracle_expt* e);
 int oracle_run(oracle_t* S, int64_t nsteps);
+int oracle_run_save(oracle_t* S);
+int oracle_adjoint(oracle_t* S1, oracle_t* S2, float* g_invK, float* g_rho);
+int oracle_set_wavelets(oracle_t* S, int q, const float* w, int64_t n);
+int oracle_get_store(const oracle_t* S, int q, int axis, float* out, int64_t n);
+int oracle_get_snap(const oracle_t* S, int f, float* out, int64_t n);
 int oracle_get_field(const oracle_t* S, int f, float* out, int64_t n);
 int oracle_set_field(oracle_t* S, int f, const float* in, int64_t n);
 int oracle_derivative(oracle_t* S, int df, int with_cpml);
 int oracle_get_memory(const oracle_t* S, int f, float* out, int64_t n);
 int oracle_get_coef(const oracle_t* S, int c, float* out, int64_t n);
 int oracle_coef_shape(const oracle_t* S, int c, int64_t out[3]);
+int oracle_set_coef(oracle_t* S, int c, const float* in, int64_t n);
 int oracle_get_pml(const oracle_t* S, int f, float* a, float* b, float* kI);
 int oracle_get_weights(const oracle_t* S, int src, int q, int64_t* idx, float* w);
 int oracle_get_records(const oracle_t* S, int q, float* out, int64_t n);

@@ -62,7 +62,13 @@ def load():
         lib.oracle_run.argtypes = [C.c_void_p, C.c_int64]
         for name in ("oracle_get_field", "oracle_get_memory", "oracle_get_coef", "oracle_get_records"):
             getattr(lib, name).argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_int64]
+        lib.oracle_run_save.argtypes = [C.c_void_p]
+        lib.oracle_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        lib.oracle_set_wavelets.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_int64]
+        lib.oracle_get_store.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int64]
+        lib.oracle_get_snap.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_int64]
         lib.oracle_set_field.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_int64]
+        lib.oracle_set_coef.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_int64]
         lib.oracle_derivative.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.oracle_coef_shape.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64)]
         lib.oracle_get_pml.argtypes = [C.c_void_p, C.c_int] + [C.POINTER(C.c_float)] * 3
@@ -181,11 +187,45 @@ class Oracle:
         assert rc == 0, f
         return out
 
+    def run_save(self):
+        """:forward_save: forward run keeping the time-reversal store and snapshots."""
+        assert load().oracle_run_save(self.h) == 0
+
+    def set_wavelets(self, q, w):
+        w = np.asfortranarray(np.asarray(w, np.float32).reshape(self.nt, self.ns, order="F"))
+        assert load().oracle_set_wavelets(self.h, q, w.ctypes.data_as(C.POINTER(C.c_float)), w.size) == 0
+
+    def adjoint(self, adj: "Oracle"):
+        """:adjoint with npw = 2: returns (g_invK, g_rho) on the stress grid (FP32)."""
+        s = self._shape("p")
+        gk = np.zeros(s, np.float32, order="F")
+        gr = np.zeros(s, np.float32, order="F")
+        rc = load().oracle_adjoint(self.h, adj.h, gk.ctypes.data_as(C.POINTER(C.c_float)),
+                                   gr.ctypes.data_as(C.POINTER(C.c_float)))
+        assert rc == 0
+        return gk, gr
+
+    def snap(self, f):
+        out = np.zeros(self._shape(f), np.float32, order="F")
+        assert load().oracle_get_snap(self.h, FID[f], out.ctypes.data_as(C.POINTER(C.c_float)), out.size) == 0
+        return out
+
+    def store(self, q, axis, shape):
+        out = np.zeros(shape, np.float32, order="F")
+        assert load().oracle_get_store(self.h, q, axis, out.ctypes.data_as(C.POINTER(C.c_float)), out.size) == 0
+        return out
+
     def set_field(self, f, a):
         a = np.asfortranarray(a, dtype=np.float32)
         assert a.shape == self._shape(f), (a.shape, self._shape(f))
         rc = load().oracle_set_field(self.h, FID[f], a.ctypes.data_as(C.POINTER(C.c_float)), a.size)
         assert rc == 0
+
+    def set_coef(self, c, a):
+        """Overwrite coefficient ``c`` (``"rho"`` is the padded density of the sources)."""
+        a = np.asfortranarray(a, dtype=np.float32)
+        rc = load().oracle_set_coef(self.h, CID[c], a.ctypes.data_as(C.POINTER(C.c_float)), a.size)
+        assert rc == 0, c
 
     def derivative(self, df, with_cpml=False):
         assert load().oracle_derivative(self.h, FID[df], int(with_cpml)) == 0

@@ -98,3 +98,105 @@ def experiment3d(medium, *, nt=120, sfields=("p",), rfields=("p", "vx", "vy", "v
     return G.SeisForwExpt(attrib, medium=medium, tgrid=tgrid, ageom=[ag], srcwav=[src],
                           rfields=list(rfields), order=order, npml=npml, compat=compat,
                           stressfree_faces=list(stressfree_faces), context=ctx, **kw)
+
+
+# ---------------------------------------------------------------------------------------
+# The reference gradient flow (src/fdtd/func_grad.jl:11-49) on the oracle
+# ---------------------------------------------------------------------------------------
+
+ORACLE_COEFS = ("dtinvavxirho", "dtinvavzirho", "dtK", "rho")
+
+
+def oracle_pair(pa, iss, adj_wavelets=None):
+    """Oracles of the two wavefields of supersource ``iss`` of a 2-D acoustic npw = 2
+    experiment, on ``pa``'s coefficient arrays; the second sprays at the receivers."""
+    import oracle as O
+    from geophyinv.fdtd import get_source
+
+    a = pa.ageom[iss]
+    s1 = O.from_experiment(pa, iss, freqpeak=float(pa.pml_freqpeak))
+    for c in ORACLE_COEFS:
+        s1.set_coef(c, pa.coefs[c])
+    vfields = [f for f in pa.rfields if f in ("vx", "vy", "vz")]
+    nt = len(pa.tgrid)
+    if adj_wavelets is None:
+        adj_wavelets = {f: np.zeros((nt, a.nr), np.float32) for f in vfields}
+    med = pa.medium
+    s2 = O.Oracle(physics=0, order=pa.order, grid=med.grid, vp=med.vp.m, rho=med.rho.m, npml=pa.npml,
+                  pml_faces=pa.pml_faces, rigid_faces=pa.rigid_faces, nt=nt, dt=pa.tgrid.step,
+                  src_pos=a.rec_points(), src_fields=vfields, wavelets=[adj_wavelets[f] for f in vfields],
+                  rec_pos=a.rec_points(), rec_fields=vfields, freqpeak=float(pa.pml_freqpeak))
+    for c in ORACLE_COEFS:
+        s2.set_coef(c, pa.coefs[c])
+    return s1, s2, get_source
+
+
+def oracle_gradient(pa, dobs):
+    """(J, g_invK, g_rho, dcal) of the reference flow for every supersource, on the oracle:
+    forward_save, adjoint sources 2 (dobs - dcal) reversed in time, type -1 wavelets for the
+    first wavefield, :adjoint run, FP32 stacking over supersources."""
+    g_k = np.zeros(pa.ext_shape, np.float32, order="F")
+    g_r = np.zeros(pa.ext_shape, np.float32, order="F")
+    J = 0.0
+    dcal = []
+    for iss, a in enumerate(pa.ageom):
+        s1, _, get_source = oracle_pair(pa, iss)
+        s1.run_save()
+        rec = {f: s1.records(q) for q, f in enumerate(pa.rfields)}
+        dcal.append(rec)
+        adj = {}
+        for f in pa.rfields:
+            d = dobs[iss].d[f].astype(np.float64)
+            J += float(np.sum((d - rec[f].astype(np.float64)) ** 2))
+            if f in ("vx", "vy", "vz"):
+                adj[f] = (2.0 * (d - rec[f].astype(np.float64))).astype(np.float32)[::-1].copy()
+        _, s2, _ = oracle_pair(pa, iss, adj)
+        for q, f in enumerate(pa.srcwav[iss].fields):
+            s1.set_wavelets(q, get_source(pa.srcwav[iss].d[f], f, -1))
+        gk, gr = s1.adjoint(s2)
+        g_k = g_k + gk
+        g_r = g_r + gr
+    return J, g_k, g_r, dcal
+
+
+def fwi_experiment(medium, *, nss=2, nt=300, sfields=("vz",), rfields=("vz",), nr=9, order=2, npml=10,
+                   pml_faces=None, mode="forward_save", ctx=None, seed=5):
+    """A 2-D acoustic npw = 2 experiment with ``nss`` supersources (one source each) and a
+    line of receivers below them."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    gz, gx = medium.grid
+    Lz, Lx = gz.last, gx.last
+    dt = 0.4 * gz.step / (1.6 * float(medium.vp.m.max()))
+    tgrid = Grid(0.0, dt, nt)
+    ageom, srcwav = [], []
+    for k in range(nss):
+        sx = Lx * (0.3 + 0.4 * k / max(nss - 1, 1)) + rng.uniform(-1, 1) * gx.step
+        sz = 0.2 * Lz + rng.uniform(-1, 1) * gz.step
+        ag = G.AGeomss({"z": [sz], "x": [sx]},
+                       {"z": np.full(nr, 0.75 * Lz) + rng.uniform(0, 1, nr) * gz.step,
+                        "x": np.linspace(0.1 * Lx, 0.9 * Lx, nr)})
+        s = G.Srcs(1, tgrid, list(sfields))
+        s.update(list(sfields), wavelet(nt, dt, 25.0, 0.03 + 0.002 * k) * 1e3)
+        ageom.append(ag)
+        srcwav.append(s)
+    kw = {} if pml_faces is None else {"pml_faces": pml_faces}
+    return G.SeisForwExpt(G.FdtdAcoustic(mode, 2), medium=medium, tgrid=tgrid, ageom=ageom, srcwav=srcwav,
+                          rfields=list(rfields), order=order, npml=npml, context=ctx, **kw)
+
+
+def oracle_records(medium, pa_like):
+    """Oracle records (one Recs per supersource) of ``pa_like``'s geometry and wavelets in
+    another medium: the observed data of gradient tests."""
+    import oracle as O
+    out = []
+    for iss in range(len(pa_like.ageom)):
+        a, s = pa_like.ageom[iss], pa_like.srcwav[iss]
+        o = O.Oracle(physics=0, order=pa_like.order, grid=medium.grid, vp=medium.vp.m, rho=medium.rho.m,
+                     npml=pa_like.npml, pml_faces=pa_like.pml_faces, rigid_faces=pa_like.rigid_faces,
+                     nt=len(pa_like.tgrid), dt=pa_like.tgrid.step, src_pos=a.src_points(),
+                     src_fields=s.fields, wavelets=[s.d[f] for f in s.fields], rec_pos=a.rec_points(),
+                     rec_fields=pa_like.rfields)
+        o.run()
+        out.append(G.Recs(a.nr, pa_like.tgrid, pa_like.rfields,
+                          {f: o.records(q) for q, f in enumerate(pa_like.rfields)}))
+    return out
