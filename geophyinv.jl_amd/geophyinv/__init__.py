@@ -9,6 +9,7 @@ from . import func_grad as _func_grad
 from .database import AGeomss, L2DistLoss, Records, Recs, SrcWav, Srcs
 from .fdtd import FdtdAcoustic, FdtdElastic, SeisForwExpt, update
 from .fields import FIELD_ID, FIELDS, field_shape, get_mgrid
+from .fwi import SeisInvExpt
 from .grid import Grid
 from .media import AcousticMedium, ElasticMedium, Medium, padarray
 from .wavelets import findfreq, ricker
@@ -34,6 +35,6 @@ def gradient(*args, **kw):
 
 __all__ = [
     "AGeom", "AGeomss", "AcousticMedium", "ElasticMedium", "FIELDS", "FIELD_ID", "FdtdAcoustic",
-    "FdtdElastic", "Grid", "L2DistLoss", "Medium", "Records", "Recs", "SeisForwExpt", "SrcWav", "Srcs",
+    "FdtdElastic", "Grid", "L2DistLoss", "Medium", "Records", "Recs", "SeisForwExpt", "SeisInvExpt", "SrcWav", "Srcs",
     "field_shape", "findfreq", "get_mgrid", "gradient", "lossvalue", "padarray", "ricker", "update",
 ]
