@@ -39,10 +39,39 @@ struct StrParams2D {
     float dxI, dzI;
 };
 
-// workgroup grid of each pass (64 x 4 threads per workgroup, workgroup b covers rows
-// 64*(b % gx).. and columns 4*(b / gx)..); returns gx*gy
-int velocity_blocks_2d(const Model& m, int* gx, int* gy);
-int stress_blocks_2d(const Model& m, int* gx, int* gy);
+// Injection buckets (64-row segment of one derivative-grid column): bucket of point (i, j)
+// = i/64 + gz*j; returns the number of buckets.  Velocity pass: (i, j) = (iz-h, ix-h) of
+// the interior velocity cell; stress pass: (iz, ix).
+int velocity_buckets_2d(const Model& m, int* gz);
+int stress_buckets_2d(const Model& m, int* gz);
+
+// Workgroup shape of a batched launch: 64 x ty threads, rx columns per thread.
+struct TileShape {
+    int ty, rx;
+};
+constexpr int NTILE_SHAPES = 4;
+extern const TileShape TILE_SHAPES[NTILE_SHAPES];
+void velocity_grid_2d(const Model& m, const TileShape& t, int* gx, int* gy);
+void stress_grid_2d(const Model& m, const TileShape& t, int* gx, int* gy);
+
+// Pass parameters of a model and state (gx, gy left for the launcher); false when the state
+// lacks an array the pass writes.
+bool vel_params_2d(const Model& m, const State& st, VelParams2D* out);
+bool str_params_2d(const Model& m, const State& st, StrParams2D* out);
+
+// One resident shot of a batched launch (device table entry).
+struct BatchShot2D {
+    VelParams2D vp;
+    StrParams2D sp;
+    PassExtras xv, xs;
+    int64_t start;      // batch step at which the shot's it = 0
+    int64_t nt;
+};
+
+// One pass of every shot of a device table: first[s] = first workgroup of shot s,
+// first[n] = nblocks; all shots share physics, order and tile shape.
+void batch_pass_2d(int physics, int order, const TileShape& t, bool velocity, const BatchShot2D* D,
+                   const int* first, int n, int nblocks, int64_t step, hipStream_t s);
 
 // The pass launchers return -1 without launching when the state lacks an array the pass
 // writes (a wavefield, or the CPML memory of an active derivative).

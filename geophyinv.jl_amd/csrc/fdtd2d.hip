@@ -82,22 +82,18 @@ __device__ __forceinline__ float dx_at(const float* a, int off, int ld)
 }
 
 // ---------------------------------------------------------------------------------------
-// acoustic
+// Points.  A thread owns derivative-grid point (i, j); `bk` is its injection bucket (the
+// 64-row segment of column j), `it` the shot's time step and `ninj` 0 when the pass takes
+// no sources (the trailing half step).
 // ---------------------------------------------------------------------------------------
 
 template <int O>
-__global__ __launch_bounds__(256) void vel2d_acoustic(VelParams2D P, PassExtras X)
+__device__ __forceinline__ void vel_ac_point(const VelParams2D& P, const PassExtras& X, int64_t it, int ninj, int i,
+                                             int j)
 {
     constexpr int H = O - 1;
-    const int b = blockIdx.x;
-    const int nbm = P.gx * P.gy;
-    if (b >= nbm) {
-        gather_block(X, b - nbm);
-        return;
-    }
-    const int i = (b % P.gx) * 64 + threadIdx.x;
-    const int j = (b / P.gx) * 4 + threadIdx.y;
     const int ld = P.ld;
+    const int bk = (i >> 6) + P.gx * j;
     // vx[inn] += dtinvavxirho * dpdx, dpdx = D_xi(p) * dxI on (nz-2h) x (nx-h)
     if (i < P.nz - 2 * H && j < P.nx - H) {
         float d = dx_at<O>(P.s0, (i + H) + ld * j, ld) * P.dxI;
@@ -108,7 +104,7 @@ __global__ __launch_bounds__(256) void vel2d_acoustic(VelParams2D P, PassExtras 
         // (dirichlet.jl ighostmin/ighostmax with ifd = 1), taken before the source term
         if ((P.rigid_mask & GPI_FACE(2, 0)) && j == 0) P.vx[(i + H)] = -nv;
         if ((P.rigid_mask & GPI_FACE(2, 1)) && j == P.nx - H - 1) P.vx[(i + H) + ld * (P.nx + H - 1)] = -nv;
-        if (X.ninj) nv = inject_velocity(X, INJ_VX, b, o, nv, P.rho, P.dt);
+        if (ninj) nv = inject_velocity(X, it, INJ_VX, bk, o, nv, P.rho, P.dt);
         P.vx[o] = nv;
     }
     // vz[inn] += dtinvavzirho * dpdz, dpdz = D_zi(p) * dzI on (nz-h) x (nx-2h)
@@ -119,22 +115,15 @@ __global__ __launch_bounds__(256) void vel2d_acoustic(VelParams2D P, PassExtras 
         float nv = P.vz[o] + P.cz[i + ld * j] * d;
         if ((P.rigid_mask & GPI_FACE(0, 0)) && i == 0) P.vz[ld * (j + H)] = -nv;
         if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1) P.vz[(P.nz + H - 1) + ld * (j + H)] = -nv;
-        if (X.ninj) nv = inject_velocity(X, INJ_VZ, b, o, nv, P.rho, P.dt);
+        if (ninj) nv = inject_velocity(X, it, INJ_VZ, bk, o, nv, P.rho, P.dt);
         P.vz[o] = nv;
     }
 }
 
 template <int O>
-__global__ __launch_bounds__(256) void str2d_acoustic(StrParams2D P, PassExtras X)
+__device__ __forceinline__ void str_ac_point(const StrParams2D& P, const PassExtras& X, int64_t it, int ninj, int i,
+                                             int j)
 {
-    const int b = blockIdx.x;
-    const int nbm = P.gx * P.gy;
-    if (b >= nbm) {
-        gather_block(X, b - nbm);
-        return;
-    }
-    const int i = (b % P.gx) * 64 + threadIdx.x;
-    const int j = (b / P.gx) * 4 + threadIdx.y;
     if (i >= P.nz || j >= P.nx) return;
     const int ld = P.ld;
     const int o = i + ld * j;
@@ -145,31 +134,22 @@ __global__ __launch_bounds__(256) void str2d_acoustic(StrParams2D P, PassExtras 
     const float c = P.c0[o];
     float p = P.s0[o] + (dvx + dvz) * c;
     // p = p + pv * dtK per source field (muladd_tauii!, source.jl:160-163)
-    for (int q = 0; q < X.ninj; ++q) {
+    const int bk = (i >> 6) + P.gx * j;
+    for (int q = 0; q < ninj; ++q) {
         float acc;
         int t;
-        if (X.inj[q].kind == INJ_P && injection_sum(X.inj[q], b, o, X.it, acc, t)) p = p + (acc * c);
+        if (X.inj[q].kind == INJ_P && injection_sum(X.inj[q], bk, o, it, acc, t)) p = p + (acc * c);
     }
     P.s0[o] = p;
 }
 
-// ---------------------------------------------------------------------------------------
-// elastic (P-SV)
-// ---------------------------------------------------------------------------------------
-
 template <int O>
-__global__ __launch_bounds__(256) void vel2d_elastic(VelParams2D P, PassExtras X)
+__device__ __forceinline__ void vel_el_point(const VelParams2D& P, const PassExtras& X, int64_t it, int ninj, int i,
+                                             int j)
 {
     constexpr int H = O - 1;
-    const int b = blockIdx.x;
-    const int nbm = P.gx * P.gy;
-    if (b >= nbm) {
-        gather_block(X, b - nbm);
-        return;
-    }
-    const int i = (b % P.gx) * 64 + threadIdx.x;
-    const int j = (b / P.gx) * 4 + threadIdx.y;
     const int ld = P.ld;
+    const int bk = (i >> 6) + P.gx * j;
     // vx[inn] -= dtinvavxirho*(dtauxxdx + dtauxzdz) on (nz-2h) x (nx-h)
     if (i < P.nz - 2 * H && j < P.nx - H) {
         float d1 = dx_at<O>(P.s0, (i + H) + ld * j, ld) * P.dxI;   // D_xi(tauxx)
@@ -180,7 +160,7 @@ __global__ __launch_bounds__(256) void vel2d_elastic(VelParams2D P, PassExtras X
         float nv = P.vx[o] - P.cx[i + ld * j] * (d1 + d2);
         if ((P.rigid_mask & GPI_FACE(2, 0)) && j == 0) P.vx[(i + H)] = -nv;
         if ((P.rigid_mask & GPI_FACE(2, 1)) && j == P.nx - H - 1) P.vx[(i + H) + ld * (P.nx + H - 1)] = -nv;
-        if (X.ninj) nv = inject_velocity(X, INJ_VX, b, o, nv, P.rho, P.dt);
+        if (ninj) nv = inject_velocity(X, it, INJ_VX, bk, o, nv, P.rho, P.dt);
         P.vx[o] = nv;
     }
     // vz[inn] -= dtinvavzirho*(dtauxzdx + dtauzzdz) on (nz-h) x (nx-2h)
@@ -193,23 +173,16 @@ __global__ __launch_bounds__(256) void vel2d_elastic(VelParams2D P, PassExtras X
         float nv = P.vz[o] - P.cz[i + ld * j] * (d1 + d2);
         if ((P.rigid_mask & GPI_FACE(0, 0)) && i == 0) P.vz[ld * (j + H)] = -nv;
         if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1) P.vz[(P.nz + H - 1) + ld * (j + H)] = -nv;
-        if (X.ninj) nv = inject_velocity(X, INJ_VZ, b, o, nv, P.rho, P.dt);
+        if (ninj) nv = inject_velocity(X, it, INJ_VZ, bk, o, nv, P.rho, P.dt);
         P.vz[o] = nv;
     }
 }
 
 template <int O>
-__global__ __launch_bounds__(256) void str2d_elastic(StrParams2D P, PassExtras X)
+__device__ __forceinline__ void str_el_point(const StrParams2D& P, const PassExtras& X, int64_t it, int ninj, int i,
+                                             int j)
 {
     constexpr int H = O - 1;
-    const int b = blockIdx.x;
-    const int nbm = P.gx * P.gy;
-    if (b >= nbm) {
-        gather_block(X, b - nbm);
-        return;
-    }
-    const int i = (b % P.gx) * 64 + threadIdx.x;
-    const int j = (b / P.gx) * 4 + threadIdx.y;
     const int ld = P.ld;
     const bool fs = (P.free_mask & GPI_FACE(0, 0)) != 0;
     if (i < P.nz && j < P.nx) {
@@ -224,10 +197,11 @@ __global__ __launch_bounds__(256) void str2d_elastic(StrParams2D P, PassExtras X
         // free surface at zmin, before the sources: tauzz[1,:] = -tauzz[2,:]
         if (fs && i == 1) P.s1[ld * j] = -tzz;
         // tauxx += pv*dtM and tauzz += pv*dtM per source field (source.jl:79-97)
-        for (int q = 0; q < X.ninj; ++q) {
+        const int bk = (i >> 6) + P.gx * j;
+        for (int q = 0; q < ninj; ++q) {
             float acc;
             int t;
-            if (X.inj[q].kind == INJ_NORMAL2 && injection_sum(X.inj[q], b, o, X.it, acc, t)) {
+            if (X.inj[q].kind == INJ_NORMAL2 && injection_sum(X.inj[q], bk, o, it, acc, t)) {
                 txx = txx + (acc * m);
                 tzz = tzz + (acc * m);
             }
@@ -247,45 +221,212 @@ __global__ __launch_bounds__(256) void str2d_elastic(StrParams2D P, PassExtras X
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Tiles: a workgroup of 64 x TY threads covers 64 rows and TY*RX columns; each thread
+// updates RX consecutive columns.  Workgroup b covers rows 64*(b % gx).. and columns
+// TY*RX*(b / gx)..
+// ---------------------------------------------------------------------------------------
+
+template <int O, int PHYS, int TY, int RX>
+__device__ __forceinline__ void vel_tile(const VelParams2D& P, const PassExtras& X, int64_t it, int ninj, int b)
+{
+    const int i = (b % P.gx) * 64 + (int)threadIdx.x;
+    const int j0 = (b / P.gx) * (TY * RX) + (int)threadIdx.y * RX;
+#pragma unroll
+    for (int r = 0; r < RX; ++r) {
+        if (PHYS == 0) vel_ac_point<O>(P, X, it, ninj, i, j0 + r);
+        else vel_el_point<O>(P, X, it, ninj, i, j0 + r);
+    }
+}
+
+template <int O, int PHYS, int TY, int RX>
+__device__ __forceinline__ void str_tile(const StrParams2D& P, const PassExtras& X, int64_t it, int ninj, int b)
+{
+    const int i = (b % P.gx) * 64 + (int)threadIdx.x;
+    const int j0 = (b / P.gx) * (TY * RX) + (int)threadIdx.y * RX;
+#pragma unroll
+    for (int r = 0; r < RX; ++r) {
+        if (PHYS == 0) str_ac_point<O>(P, X, it, ninj, i, j0 + r);
+        else str_el_point<O>(P, X, it, ninj, i, j0 + r);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// One shot per launch (the drop-in path): 64 x 4 workgroups, one column per thread.
+// ---------------------------------------------------------------------------------------
+
+constexpr int PLAIN_TY = 4;
+constexpr int PLAIN_RX = 1;
+
+template <int O, int PHYS>
+__global__ __launch_bounds__(64 * PLAIN_TY) void vel2d_kernel(VelParams2D P, PassExtras X)
+{
+    const int b = blockIdx.x;
+    const int nbm = P.gx * P.gy;
+    if (b >= nbm) {
+        gather_block(X, X.it, b - nbm);
+        return;
+    }
+    vel_tile<O, PHYS, PLAIN_TY, PLAIN_RX>(P, X, X.it, X.ninj, b);
+}
+
+template <int O, int PHYS>
+__global__ __launch_bounds__(64 * PLAIN_TY) void str2d_kernel(StrParams2D P, PassExtras X)
+{
+    const int b = blockIdx.x;
+    const int nbm = P.gx * P.gy;
+    if (b >= nbm) {
+        gather_block(X, X.it, b - nbm);
+        return;
+    }
+    str_tile<O, PHYS, PLAIN_TY, PLAIN_RX>(P, X, X.it, X.ninj, b);
+}
+
 template <int O>
 void launch_vel(const Model& m, hipStream_t s, VelParams2D P, const PassExtras& X)
 {
     constexpr int H = O - 1;
     P.gx = (P.nz - H + 63) / 64;
-    P.gy = (P.nx - H + 3) / 4;
-    dim3 grd(P.gx * P.gy + X.gather_blocks), blk(64, 4);
-    if (m.g.physics == 0) hipLaunchKernelGGL(vel2d_acoustic<O>, grd, blk, 0, s, P, X);
-    else hipLaunchKernelGGL(vel2d_elastic<O>, grd, blk, 0, s, P, X);
+    P.gy = (P.nx - H + PLAIN_TY * PLAIN_RX - 1) / (PLAIN_TY * PLAIN_RX);
+    dim3 grd(P.gx * P.gy + X.gather_blocks), blk(64, PLAIN_TY);
+    if (m.g.physics == 0) hipLaunchKernelGGL((vel2d_kernel<O, 0>), grd, blk, 0, s, P, X);
+    else hipLaunchKernelGGL((vel2d_kernel<O, 1>), grd, blk, 0, s, P, X);
 }
 
 template <int O>
 void launch_str(const Model& m, hipStream_t s, StrParams2D P, const PassExtras& X)
 {
     P.gx = (P.nz + 63) / 64;
-    P.gy = (P.nx + 3) / 4;
-    dim3 grd(P.gx * P.gy + X.gather_blocks), blk(64, 4);
-    if (m.g.physics == 0) hipLaunchKernelGGL(str2d_acoustic<O>, grd, blk, 0, s, P, X);
-    else hipLaunchKernelGGL(str2d_elastic<O>, grd, blk, 0, s, P, X);
+    P.gy = (P.nx + PLAIN_TY * PLAIN_RX - 1) / (PLAIN_TY * PLAIN_RX);
+    dim3 grd(P.gx * P.gy + X.gather_blocks), blk(64, PLAIN_TY);
+    if (m.g.physics == 0) hipLaunchKernelGGL((str2d_kernel<O, 0>), grd, blk, 0, s, P, X);
+    else hipLaunchKernelGGL((str2d_kernel<O, 1>), grd, blk, 0, s, P, X);
+}
+
+// ---------------------------------------------------------------------------------------
+// Many shots per launch (batched serving).  The launch covers every resident shot's
+// workgroups back to back; first[s] is shot s's first workgroup (first[n] the total).
+// Each shot advances at its own time step it = step - start: shots past their last step
+// idle, and at it == nt a shot takes the trailing velocity half step of
+// src/fdtd/propagate.jl:254-255 (no sources, no receivers).
+// ---------------------------------------------------------------------------------------
+
+__device__ __forceinline__ int shot_of_block(const int* __restrict__ first, int n, int b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+template <int O, int PHYS, int TY, int RX>
+__global__ __launch_bounds__(64 * TY) void vel2d_batch(const BatchShot2D* __restrict__ D, const int* __restrict__ first,
+                                                       int n, int64_t step)
+{
+    const int b = blockIdx.x;
+    const int s = shot_of_block(first, n, b);
+    const BatchShot2D& S = D[s];
+    const int64_t it = step - S.start;
+    if (it < 0 || it > S.nt) return;
+    const int lb = b - first[s];
+    const int nbm = S.vp.gx * S.vp.gy;
+    if (lb >= nbm) {
+        if (it < S.nt) gather_block(S.xv, it, lb - nbm);
+        return;
+    }
+    vel_tile<O, PHYS, TY, RX>(S.vp, S.xv, it, it < S.nt ? S.xv.ninj : 0, lb);
+}
+
+template <int O, int PHYS, int TY, int RX>
+__global__ __launch_bounds__(64 * TY) void str2d_batch(const BatchShot2D* __restrict__ D, const int* __restrict__ first,
+                                                       int n, int64_t step)
+{
+    const int b = blockIdx.x;
+    const int s = shot_of_block(first, n, b);
+    const BatchShot2D& S = D[s];
+    const int64_t it = step - S.start;
+    if (it < 0 || it >= S.nt) return;
+    const int lb = b - first[s];
+    const int nbm = S.sp.gx * S.sp.gy;
+    if (lb >= nbm) {
+        gather_block(S.xs, it, lb - nbm);
+        return;
+    }
+    str_tile<O, PHYS, TY, RX>(S.sp, S.xs, it, S.xs.ninj, lb);
+}
+
+template <int O, int PHYS, int TY, int RX>
+void launch_batch_t(bool velocity, const BatchShot2D* D, const int* first, int n, int nblocks, int64_t step,
+                    hipStream_t s)
+{
+    dim3 grd(nblocks), blk(64, TY);
+    if (velocity) hipLaunchKernelGGL((vel2d_batch<O, PHYS, TY, RX>), grd, blk, 0, s, D, first, n, step);
+    else hipLaunchKernelGGL((str2d_batch<O, PHYS, TY, RX>), grd, blk, 0, s, D, first, n, step);
+}
+
+template <int O, int PHYS>
+void launch_batch_o(const TileShape& t, bool velocity, const BatchShot2D* D, const int* first, int n, int nblocks,
+                    int64_t step, hipStream_t s)
+{
+    if (t.ty == 4 && t.rx == 1) launch_batch_t<O, PHYS, 4, 1>(velocity, D, first, n, nblocks, step, s);
+    else if (t.ty == 8 && t.rx == 1) launch_batch_t<O, PHYS, 8, 1>(velocity, D, first, n, nblocks, step, s);
+    else if (t.ty == 2 && t.rx == 2) launch_batch_t<O, PHYS, 2, 2>(velocity, D, first, n, nblocks, step, s);
+    else launch_batch_t<O, PHYS, 4, 2>(velocity, D, first, n, nblocks, step, s);
 }
 
 }  // namespace
 
-int velocity_blocks_2d(const Model& m, int* gx, int* gy)
+const TileShape TILE_SHAPES[NTILE_SHAPES] = {{4, 1}, {8, 1}, {2, 2}, {4, 2}};
+
+int velocity_buckets_2d(const Model& m, int* gz)
 {
-    const int H = m.h;
-    *gx = (int)((m.g.n[0] - H + 63) / 64);
-    *gy = (int)((m.g.n[2] - H + 3) / 4);
-    return *gx * *gy;
+    *gz = (int)((m.g.n[0] - m.h + 63) / 64);
+    return *gz * (int)(m.g.n[2] - m.h);
 }
 
-int stress_blocks_2d(const Model& m, int* gx, int* gy)
+int stress_buckets_2d(const Model& m, int* gz)
+{
+    *gz = (int)((m.g.n[0] + 63) / 64);
+    return *gz * (int)m.g.n[2];
+}
+
+void velocity_grid_2d(const Model& m, const TileShape& t, int* gx, int* gy)
+{
+    *gx = (int)((m.g.n[0] - m.h + 63) / 64);
+    *gy = (int)((m.g.n[2] - m.h + t.ty * t.rx - 1) / (t.ty * t.rx));
+}
+
+void stress_grid_2d(const Model& m, const TileShape& t, int* gx, int* gy)
 {
     *gx = (int)((m.g.n[0] + 63) / 64);
-    *gy = (int)((m.g.n[2] + 3) / 4);
-    return *gx * *gy;
+    *gy = (int)((m.g.n[2] + t.ty * t.rx - 1) / (t.ty * t.rx));
 }
 
-int velocity_pass_2d(const Model& m, const State& st, hipStream_t s, const PassExtras& X)
+void batch_pass_2d(int physics, int order, const TileShape& t, bool velocity, const BatchShot2D* D, const int* first,
+                   int n, int nblocks, int64_t step, hipStream_t s)
+{
+    if (nblocks <= 0) return;
+    if (physics == 0) {
+        switch (order) {
+        case 2: launch_batch_o<2, 0>(t, velocity, D, first, n, nblocks, step, s); break;
+        case 4: launch_batch_o<4, 0>(t, velocity, D, first, n, nblocks, step, s); break;
+        case 6: launch_batch_o<6, 0>(t, velocity, D, first, n, nblocks, step, s); break;
+        default: launch_batch_o<8, 0>(t, velocity, D, first, n, nblocks, step, s); break;
+        }
+    } else {
+        switch (order) {
+        case 2: launch_batch_o<2, 1>(t, velocity, D, first, n, nblocks, step, s); break;
+        case 4: launch_batch_o<4, 1>(t, velocity, D, first, n, nblocks, step, s); break;
+        case 6: launch_batch_o<6, 1>(t, velocity, D, first, n, nblocks, step, s); break;
+        default: launch_batch_o<8, 1>(t, velocity, D, first, n, nblocks, step, s); break;
+        }
+    }
+}
+
+bool vel_params_2d(const Model& m, const State& st, VelParams2D* out)
 {
     VelParams2D P{};
     const bool ac = m.g.physics == 0;
@@ -311,17 +452,13 @@ int velocity_pass_2d(const Model& m, const State& st, hipStream_t s, const PassE
     P.dzI = m.g.dsI[0];
     P.dt = m.g.dt;
     const Pml* q[4] = {&P.qx0, &P.qx1, &P.qz0, &P.qz1};
-    if (!state_complete(m, q, 4) || !P.s0 || !P.vx || !P.vz || (!ac && (!P.s1 || !P.s2))) return -1;
-    switch (m.g.order) {
-    case 2: launch_vel<2>(m, s, P, X); break;
-    case 4: launch_vel<4>(m, s, P, X); break;
-    case 6: launch_vel<6>(m, s, P, X); break;
-    default: launch_vel<8>(m, s, P, X); break;
-    }
-    return 0;
+    if (!state_complete(m, q, 4) || !P.s0 || !P.vx || !P.vz || !P.cx || !P.cz || (!ac && (!P.s1 || !P.s2)))
+        return false;
+    *out = P;
+    return true;
 }
 
-int stress_pass_2d(const Model& m, const State& st, hipStream_t s, const PassExtras& X)
+bool str_params_2d(const Model& m, const State& st, StrParams2D* out)
 {
     StrParams2D P{};
     const bool ac = m.g.physics == 0;
@@ -346,7 +483,29 @@ int stress_pass_2d(const Model& m, const State& st, hipStream_t s, const PassExt
     P.dxI = m.g.dsI[2];
     P.dzI = m.g.dsI[0];
     const Pml* q[4] = {&P.qx0, &P.qx1, &P.qz0, &P.qz1};
-    if (!state_complete(m, q, 4) || !P.s0 || !P.vx || !P.vz || (!ac && (!P.s1 || !P.s2))) return -1;
+    if (!state_complete(m, q, 4) || !P.s0 || !P.vx || !P.vz || !P.c0 || (!ac && (!P.s1 || !P.s2 || !P.c1 || !P.c2)))
+        return false;
+    *out = P;
+    return true;
+}
+
+int velocity_pass_2d(const Model& m, const State& st, hipStream_t s, const PassExtras& X)
+{
+    VelParams2D P;
+    if (!vel_params_2d(m, st, &P)) return -1;
+    switch (m.g.order) {
+    case 2: launch_vel<2>(m, s, P, X); break;
+    case 4: launch_vel<4>(m, s, P, X); break;
+    case 6: launch_vel<6>(m, s, P, X); break;
+    default: launch_vel<8>(m, s, P, X); break;
+    }
+    return 0;
+}
+
+int stress_pass_2d(const Model& m, const State& st, hipStream_t s, const PassExtras& X)
+{
+    StrParams2D P;
+    if (!str_params_2d(m, st, &P)) return -1;
     switch (m.g.order) {
     case 2: launch_str<2>(m, s, P, X); break;
     case 4: launch_str<4>(m, s, P, X); break;

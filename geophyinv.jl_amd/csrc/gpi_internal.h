@@ -32,7 +32,8 @@ struct DevPoints {
     int64_t* idx = nullptr;     // gather: npts*nnz padded-layout offsets
     float* w = nullptr;         // gather: npts*nnz weights
     int32_t ntarget = 0;        // scatter: unique target cells (interior only for velocity)
-    int32_t* blk = nullptr;     // scatter: per-workgroup target ranges of the owning pass
+    int32_t* blk = nullptr;     // scatter: target ranges per injection bucket of the owning pass
+    int32_t nbucket = 0;
     int64_t* toff = nullptr;    // scatter: padded offsets of targets (then 2 rho offsets each)
     int32_t* tptr = nullptr;    // scatter: ntarget+1 CSR row pointers
     int32_t* tsrc = nullptr;    // scatter: source column per contribution
@@ -80,6 +81,7 @@ struct Store {
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool async_alloc = false;            // allocate/free stream-ordered on `stream`
     hipEvent_t ev[4] = {};
     std::vector<hipEvent_t> sample_ev;   // 4 events per sampled step
     uint64_t budget = 0, used = 0;

@@ -33,23 +33,19 @@ int stress_pass(gpi_ctx* c, const Model& m, const State& st, const PassExtras& X
     return rc ? fail(c, GPI_EINVAL, "internal error: incomplete wavefield state") : GPI_OK;
 }
 
-// Source sets of one wavefield: fused into the passes' PassExtras when every set of a pass
-// can ride along, else applied by the stand-alone kernel after the pass (all-or-nothing per
-// pass keeps the reference's order of updates).
-struct Sources {
-    std::vector<DevPoints> pts;
-    std::vector<int> vfall, sfall;   // sets injected after the velocity / stress pass
-};
+}  // namespace
+
+namespace gpi {
 
 int plan_sources(gpi_ctx* c, const Model& m, Scratch& S, int nsrc, const gpi_points* src, const float* const* wav,
                  int64_t nt, bool velocity_only, PassExtras& XV, PassExtras& XS, Sources& out)
 {
     const gpi_grid& g = m.g;
     const int nnz_expect = g.ndim == 2 ? 4 : 8;
-    int vgx, vgy, sgx, sgy;
+    int vgx = 0, vgy = 0, sgx = 0, sgy = 0;
     const bool d3 = g.ndim == 3;
-    const int vnb = d3 ? velocity_blocks_3d(m, &vgx, &vgy) : velocity_blocks_2d(m, &vgx, &vgy);
-    const int snb = d3 ? stress_blocks_3d(m, &sgx, &sgy) : stress_blocks_2d(m, &sgx, &sgy);
+    const int vnb = d3 ? velocity_blocks_3d(m, &vgx, &vgy) : velocity_buckets_2d(m, &vgx);
+    const int snb = d3 ? stress_blocks_3d(m, &sgx, &sgy) : stress_buckets_2d(m, &sgx);
     out.pts.assign(nsrc, DevPoints{});
     std::vector<int> kinds(nsrc);
     bool vel_fused = true, str_fused = true;
@@ -93,9 +89,9 @@ int plan_sources(gpi_ctx* c, const Model& m, Scratch& S, int nsrc, const gpi_poi
                 if (!(iz >= h && iz < n3[0] - h && ix >= h && ix < n3[2] - h)) continue;
                 if (d3 && !(iy >= h && iy < n3[1] - h)) continue;
                 blk = d3 ? (int)((iz - h) / 64 + vgx * ((iy - h) / 4 + vgy * (ix - h)))
-                         : (int)((iz - h) / 64 + vgx * ((ix - h) / 4));
+                         : (int)((iz - h) / 64 + vgx * (ix - h));
             } else {
-                blk = d3 ? (int)(iz / 64 + sgx * (iy / 4 + sgy * ix)) : (int)(iz / 64 + sgx * (ix / 4));
+                blk = d3 ? (int)(iz / 64 + sgx * (iy / 4 + sgy * ix)) : (int)(iz / 64 + sgx * ix);
                 // a source on the free-surface row is applied after the surface condition
                 // by the stand-alone kernel (source.jl:61-119 runs after update_stress!)
                 if ((g.free_mask & GPI_FACE(0, 0)) && iz == 0) str_fused = false;
@@ -113,6 +109,7 @@ int plan_sources(gpi_ctx* c, const Model& m, Scratch& S, int nsrc, const gpi_poi
         }
         if (maxper > 32) (vel ? vel_fused : str_fused) = false;   // keep owner scans short
         DevPoints& D = out.pts[q];
+        D.nbucket = nb;
         D.field = f;
         D.npts = P.npts;
         D.nnz = P.nnz;
@@ -230,10 +227,14 @@ int plan_receivers(gpi_ctx* c, const Model& m, const State& st, Scratch& S, cons
         int& mx = vfield ? smax : vmax;
         mx = std::max(mx, P.npts);
     }
-    XV.gather_blocks = XV.g.nset * ((vmax + 255) / 256);
-    XS.gather_blocks = XS.g.nset * ((smax + 255) / 256);
+    XV.gather_blocks = XV.g.nset * ((vmax + GATHER_PER_BLOCK - 1) / GATHER_PER_BLOCK);
+    XS.gather_blocks = XS.g.nset * ((smax + GATHER_PER_BLOCK - 1) / GATHER_PER_BLOCK);
     return GPI_OK;
 }
+
+}  // namespace gpi
+
+namespace {
 
 // stress fields kept by boundary_save! (boundary.jl:217-264): p, or tauxx, tauxz, tauzz
 int stored_fields(const Model& m, int out[3])

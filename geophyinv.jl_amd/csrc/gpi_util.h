@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "gpi_internal.h"
+#include "pass_extras.h"
 
 struct gpi_ctx : public gpi::Ctx {};
 
@@ -60,6 +61,34 @@ inline int64_t padded(const Model& m, const int64_t n[3], int64_t lin)
     return iz + m.pitch0 * (iy + m.pitch1 * ix);
 }
 
+struct Scratch;
+
+// Source sets of one wavefield: fused into the passes' PassExtras when every set of a pass
+// can ride along, else applied by the stand-alone kernel after the pass (all-or-nothing per
+// pass keeps the reference's order of updates).  gpi_shot.hip.
+struct Sources {
+    std::vector<DevPoints> pts;
+    std::vector<int> vfall, sfall;   // sets injected after the velocity / stress pass
+};
+int plan_sources(gpi_ctx* c, const Model& m, Scratch& S, int nsrc, const gpi_points* src, const float* const* wav,
+                 int64_t nt, bool velocity_only, PassExtras& XV, PassExtras& XS, Sources& out);
+int plan_receivers(gpi_ctx* c, const Model& m, const State& st, Scratch& S, const gpi_shot* shot, PassExtras& XV,
+                   PassExtras& XS, std::vector<DevPoints>& rec);
+
+// model construction pieces (gpi_api.hip) and the device coefficient preparation
+// (medium.hip), shared by the C ABI and the serving scheduler
+int model_begin(gpi_ctx* c, const gpi_grid* g, Model** out);
+int model_pml(gpi_ctx* c, Model* m, const gpi_pml* pml, int32_t npml_arrays);
+int model_state(gpi_ctx* c, Model* m);
+int create_model_medium(gpi_ctx* c, const gpi_grid* g, const gpi_medium* med, const gpi_pml* pml,
+                        int32_t npml_arrays, Model** out);
+int prep_coefficients_2d(gpi_ctx* c, Model& m, const float* vp, const float* vs, const float* rho,
+                         const int64_t nm[3], hipStream_t s);
+
+// host -> device copy that also works on contexts with stream-ordered allocation (the
+// allocation must have completed before a blocking copy touches it)
+int copy_h2d(gpi_ctx* c, void* dst, const void* src, size_t bytes);
+
 int upload_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3]);
 int download_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3]);
 
@@ -71,7 +100,8 @@ struct Scratch {
     explicit Scratch(gpi_ctx* cc) : c(cc) {}
     ~Scratch()
     {
-        (void)hipStreamSynchronize(c->stream);
+        if (p.empty()) return;
+        if (!c->async_alloc) (void)hipStreamSynchronize(c->stream);
         free_allocs(c, p, bytes);
     }
     template <typename T>
@@ -89,10 +119,7 @@ struct Scratch {
     {
         int rc = alloc(out, v.size());
         if (rc) return rc;
-        if (!v.empty()) {
-            hipError_t e = hipMemcpy(*out, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
-            if (e != hipSuccess) return fail(c, GPI_EHIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
-        }
+        if (!v.empty()) return copy_h2d(c, *out, v.data(), sizeof(T) * v.size());
         return GPI_OK;
     }
 };

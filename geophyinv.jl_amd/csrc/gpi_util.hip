@@ -113,7 +113,8 @@ int dev_alloc(gpi_ctx* c, std::vector<void*>* owner, uint64_t* owner_bytes, void
 {
     if (c->budget && c->used + bytes > c->budget)
         return fail(c, GPI_ENOMEM, "allocation of " + std::to_string(bytes) + " bytes exceeds the context's HBM budget");
-    hipError_t e = hipMalloc(p, bytes);
+    // stream-ordered allocation on contexts that run a scheduler (no device-wide sync)
+    hipError_t e = c->async_alloc ? hipMallocAsync(p, bytes, c->stream) : hipMalloc(p, bytes);
     if (e != hipSuccess) return fail(c, GPI_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     c->used += bytes;
     owner->push_back(*p);
@@ -123,7 +124,7 @@ int dev_alloc(gpi_ctx* c, std::vector<void*>* owner, uint64_t* owner_bytes, void
 
 void free_allocs(gpi_ctx* c, std::vector<void*>& allocs, uint64_t& bytes)
 {
-    for (void* p : allocs) (void)hipFree(p);
+    for (void* p : allocs) (void)(c->async_alloc ? hipFreeAsync(p, c->stream) : hipFree(p));
     allocs.clear();
     c->used -= std::min<uint64_t>(c->used, bytes);
     bytes = 0;
@@ -178,6 +179,13 @@ int zero_state(gpi_ctx* c, const Model& m, const State& st, hipStream_t s)
     return GPI_OK;
 }
 
+int copy_h2d(gpi_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    if (c->async_alloc) HIPTRY(c, hipStreamSynchronize(c->stream));
+    HIPTRY(c, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return GPI_OK;
+}
+
 int upload_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3])
 {
     std::vector<float> stage((size_t)m.slab, 0.0f);
@@ -185,8 +193,7 @@ int upload_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, con
         for (int64_t iy = 0; iy < n[1]; ++iy)
             std::memcpy(&stage[(size_t)(m.pitch0 * (iy + m.pitch1 * ix))], src + n[0] * (iy + n[1] * ix),
                         sizeof(float) * (size_t)n[0]);
-    HIPTRY(c, hipMemcpy(dst, stage.data(), sizeof(float) * (size_t)m.slab, hipMemcpyHostToDevice));
-    return GPI_OK;
+    return copy_h2d(c, dst, stage.data(), sizeof(float) * (size_t)m.slab);
 }
 
 int download_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3])

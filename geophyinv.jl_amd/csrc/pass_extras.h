@@ -1,9 +1,11 @@
 // Work that rides along with a stencil pass: source injection by the thread that owns the
 // updated cell, and receiver sampling by extra workgroups appended to the launch.
 //
-// Injection: targets of one source field are sorted by the workgroup that owns them;
-// blk[b]..blk[b+1] lists the targets of workgroup b, so a workgroup without targets pays
-// two cached loads.  The owner sums the contributions in ascending source order (the CSC
+// Injection: targets of one source field are sorted by the bucket that owns them: in 2-D
+// a 64-row segment of one derivative-grid column (the cells one wavefront updates, so the
+// bucket is wavefront-uniform whatever the workgroup shape), in 3-D the workgroup.
+// blk[b]..blk[b+1] lists the targets of bucket b, so a bucket without targets pays two
+// uniform loads.  The owner sums the contributions in ascending source order (the CSC
 // spmv order of src/fdtd/source.jl:71,89,109,136,152) and applies the reference's update
 // expression to the value it has just computed, so the result equals the separate
 // injection pass of the reference bit for bit.
@@ -69,15 +71,15 @@ __device__ __forceinline__ bool injection_sum(const InjSet& S, int b, int64_t of
 }
 
 // v[inn] = v[inn] + (pv / @av(rho)) * dt  (muladd_with_density_v*!, source.jl:166-177)
-__device__ __forceinline__ float inject_velocity(const PassExtras& X, int kind, int b, int64_t off, float v,
-                                                 const float* rho, float dt)
+__device__ __forceinline__ float inject_velocity(const PassExtras& X, int64_t it, int kind, int b, int64_t off,
+                                                 float v, const float* rho, float dt)
 {
     for (int q = 0; q < X.ninj; ++q) {
         const InjSet& S = X.inj[q];
         if (S.kind != kind) continue;
         float acc;
         int t;
-        if (injection_sum(S, b, off, X.it, acc, t)) {
+        if (injection_sum(S, b, off, it, acc, t)) {
             const float av = (rho[S.roff[2 * t]] + rho[S.roff[2 * t + 1]]) * 0.5f;
             v = v + ((acc / av) * dt);
         }
@@ -85,19 +87,36 @@ __device__ __forceinline__ float inject_velocity(const PassExtras& X, int kind, 
     return v;
 }
 
+__device__ __forceinline__ float inject_velocity(const PassExtras& X, int kind, int b, int64_t off, float v,
+                                                 const float* rho, float dt)
+{
+    return inject_velocity(X, X.it, kind, b, off, v, rho, dt);
+}
+
+// Receivers per gather workgroup, whatever the workgroup size: gather_blocks =
+// nset * ceil(max npts / GATHER_PER_BLOCK).
+constexpr int GATHER_PER_BLOCK = 256;
+
 // receiver sampling by workgroup gb of the extra range (one lane per receiver)
-__device__ __forceinline__ void gather_block(const PassExtras& X, int gb)
+__device__ __forceinline__ void gather_block(const PassExtras& X, int64_t it, int gb)
 {
     const int per = (X.gather_blocks + X.g.nset - 1) / X.g.nset;
     const int s = gb / per;
-    const int r = (gb - s * per) * (int)(blockDim.x * blockDim.y) + (int)(threadIdx.y * blockDim.x + threadIdx.x);
-    if (s >= X.g.nset || r >= X.g.npts[s]) return;
-    const int64_t* ix = X.g.idx[s] + (int64_t)r * X.g.nnz;
-    const float* ww = X.g.w[s] + (int64_t)r * X.g.nnz;
+    if (s >= X.g.nset) return;
+    const int nth = (int)(blockDim.x * blockDim.y);
+    const int lane = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    const int base = (gb - s * per) * GATHER_PER_BLOCK;
+    const int stop = min(base + GATHER_PER_BLOCK, X.g.npts[s]);
     const float* F = X.g.field[s];
-    float tmp = 0.0f;
-    for (int k = 0; k < X.g.nnz; ++k) tmp += ww[k] * F[ix[k]];
-    X.g.rec[s][X.it * (int64_t)X.g.npts[s] + r] = tmp;
+    for (int r = base + lane; r < stop; r += nth) {
+        const int64_t* ix = X.g.idx[s] + (int64_t)r * X.g.nnz;
+        const float* ww = X.g.w[s] + (int64_t)r * X.g.nnz;
+        float tmp = 0.0f;
+        for (int k = 0; k < X.g.nnz; ++k) tmp += ww[k] * F[ix[k]];
+        X.g.rec[s][it * (int64_t)X.g.npts[s] + r] = tmp;
+    }
 }
+
+__device__ __forceinline__ void gather_block(const PassExtras& X, int gb) { gather_block(X, X.it, gb); }
 
 }  // namespace gpi

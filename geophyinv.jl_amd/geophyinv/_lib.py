@@ -60,6 +60,30 @@ class gpi_adjoint_shot(C.Structure):
                 ("src2", C.POINTER(gpi_points)), ("wav2", C.POINTER(C.POINTER(C.c_float)))]
 
 
+class gpi_medium(C.Structure):
+    _fields_ = [("n", C.c_int64 * 3), ("vp", C.POINTER(C.c_float)), ("vs", C.POINTER(C.c_float)),
+                ("rho", C.POINTER(C.c_float))]
+
+
+class gpi_server_opts(C.Structure):
+    _fields_ = [("physics", C.c_int32), ("order", C.c_int32), ("max_shots", C.c_int32),
+                ("chunk_steps", C.c_int32), ("autotune", C.c_int32), ("tile", C.c_int32),
+                ("hbm_budget_bytes", C.c_uint64)]
+
+
+class gpi_request(C.Structure):
+    _fields_ = [("grid", gpi_grid), ("medium", gpi_medium), ("pml", C.POINTER(gpi_pml)),
+                ("npml_arrays", C.c_int32), ("reserved", C.c_int32), ("shot", gpi_shot)]
+
+
+class gpi_server_stats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("active", C.c_int64), ("queued", C.c_int64), ("steps", C.c_int64), ("launches", C.c_int64),
+                ("busy_ms", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("mean_batch", C.c_double), ("tile_ty", C.c_int32), ("tile_rx", C.c_int32),
+                ("hbm_used", C.c_uint64), ("hbm_budget", C.c_uint64)]
+
+
 class gpi_timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("velocity_avg_ms", C.c_double), ("stress_avg_ms", C.c_double),
                 ("steps", C.c_int64), ("sampled", C.c_int64), ("launches", C.c_int64)]
@@ -77,6 +101,16 @@ SIGNATURES = {
                                    C.POINTER(gpi_pml), C.c_int32, C.POINTER(C.c_uint64)]),
     "gpi_model_update": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gpi_array), C.c_int32]),
     "gpi_model_free": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "gpi_model_create_medium": (C.c_int, [C.c_void_p, C.POINTER(gpi_grid), C.POINTER(gpi_medium),
+                                          C.POINTER(gpi_pml), C.c_int32, C.POINTER(C.c_uint64)]),
+    "gpi_model_get_coef": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.POINTER(C.c_float), C.c_int64]),
+    "gpi_server_create": (C.c_int, [C.c_int, C.POINTER(gpi_server_opts), C.POINTER(C.c_void_p)]),
+    "gpi_server_destroy": (None, [C.c_void_p]),
+    "gpi_server_error": (C.c_char_p, [C.c_void_p]),
+    "gpi_submit": (C.c_int, [C.c_void_p, C.POINTER(gpi_request), C.POINTER(C.c_uint64)]),
+    "gpi_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32)]),
+    "gpi_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(C.c_int32)]),
+    "gpi_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(gpi_server_stats)]),
     "gpi_forward": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gpi_shot)]),
     "gpi_forward_save": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gpi_shot), C.POINTER(C.c_uint64)]),
     "gpi_adjoint": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(gpi_adjoint_shot), C.c_float,

@@ -109,7 +109,7 @@ class SeisForwExpt:
     def __init__(self, attrib_mod: _Attrib, *, medium: Medium, tgrid: Grid, ageom, srcwav,
                  rfields=("vz",), pml_faces=None, rigid_faces=None, stressfree_faces=(),
                  order: int = 2, npml: int | None = None, compat: int = 1, verbose: bool = False,
-                 context: "_lib.Context | None" = None):
+                 context: "_lib.Context | None" = None, coefficients: str = "host"):
         ndim = medium.ndim
         if order not in (2, 4, 6, 8):
             raise ValueError("order must be 2, 4, 6 or 8")
@@ -164,11 +164,19 @@ class SeisForwExpt:
         div = {2: 1.0, 4: 24.0, 6: 1920.0, 8: 107520.0}[order]
         self.dt32 = F32(tgrid.step)
         self.dsI = [F32(1.0 / (g.step * div)) for g in self.exmedium.grid]
-        # ---- pac.mod, its reference values and pac.dmod (src/fdtd/fdtd.jl:170-176, medium.jl:131-186)
+        # ---- pac.mod, its reference values and pac.dmod (src/fdtd/fdtd.jl:170-176, medium.jl:131-186).
+        # coefficients="device" leaves the arrays to gpi_model_create_medium (2-D) and builds
+        # the host copies only if a model-vector method asks for them.
+        if coefficients not in ("host", "device"):
+            raise ValueError("coefficients must be 'host' or 'device'")
+        if coefficients == "device" and ndim != 2:
+            raise ValueError("device coefficient preparation is 2-D only")
+        self.coef_mode = coefficients
         self.mparams = list(MPARAMS["acoustic" if physics == 0 else "elastic"])
-        self.mod = independent(self.exmedium)
-        self.ref_mod = {k: MediumParameter(v).ref for k, v in self.mod.items()}
-        self.coefs = dependent(self.mod, self.dt32, order, ndim)
+        self._exmedium0 = self.exmedium
+        self._mod = self._ref_mod = self._coefs = None
+        if coefficients == "host":
+            self.coefs
         # ---- wavelets and frequency bounds (src/fdtd/source.jl:24-58, 192-246); with npw = 2
         # the second wavefield's sources sit at the receivers (fdtd.jl:90-95)
         self.npw = attrib_mod.npw
@@ -191,6 +199,30 @@ class SeisForwExpt:
         self.timing = None
 
     # ------------------------------------------------------------------ host preparation
+    @property
+    def mod(self) -> dict:
+        """``pac.mod``: independent FP32 parameters on the padded grid."""
+        if self._mod is None:
+            self._mod = independent(self.exmedium)
+        return self._mod
+
+    @property
+    def ref_mod(self) -> dict:
+        """``pac.ref_mod``: reference values of the construction-time parameters."""
+        if self._ref_mod is None:
+            base = self._mod if self.exmedium is self._exmedium0 and self._mod is not None else independent(self._exmedium0)
+            self._ref_mod = {k: MediumParameter(v).ref for k, v in base.items()}
+        return self._ref_mod
+
+    @property
+    def coefs(self) -> dict:
+        """``pac.mod`` and ``pac.dmod`` keyed by the reference's names."""
+        if self._coefs is None:
+            mod = self.mod
+            self.ref_mod
+            self._coefs = dependent(mod, self.dt32, self.order, self.ndim)
+        return self._coefs
+
     @property
     def ext_shape(self) -> tuple:
         return tuple(g.n for g in self.exmedium.grid)
@@ -289,7 +321,6 @@ class SeisForwExpt:
         if self._ctx is None:
             self._ctx = _lib.default_context()
         ctx = self._ctx
-        keep, arrs = self.coef_arrays()
         pmls = []
         for df, (a, b, k) in self.pml.items():
             p = _lib.gpi_pml()
@@ -300,8 +331,36 @@ class SeisForwExpt:
         parr = (_lib.gpi_pml * len(pmls))(*pmls)
         h = C.c_uint64()
         g = self.grid_struct()
-        ctx.check(ctx.lib.gpi_model_create(ctx.h, C.byref(g), arrs, len(arrs), parr, len(pmls), C.byref(h)))
+        if self.coef_mode == "device" and self._coefs is None:
+            med, keep = self.medium_struct()
+            ctx.check(ctx.lib.gpi_model_create_medium(ctx.h, C.byref(g), C.byref(med), parr, len(pmls), C.byref(h)))
+        else:
+            keep, arrs = self.coef_arrays()
+            ctx.check(ctx.lib.gpi_model_create(ctx.h, C.byref(g), arrs, len(arrs), parr, len(pmls), C.byref(h)))
         self._model = h.value
+
+    def medium_struct(self):
+        """``gpi_medium`` of the unpadded medium (arrays kept alive by the second value)."""
+        keep = [np.asfortranarray(getattr(self.medium, k).m, dtype=F32) for k in ("vp", "vs", "rho")
+                if hasattr(self.medium, k)]
+        med = _lib.gpi_medium()
+        shp = self.medium.vp.m.shape
+        med.n[0], med.n[1], med.n[2] = shp[0], 1, shp[-1]
+        med.vp = _lib.fptr(keep[0])
+        med.rho = _lib.fptr(keep[-1])
+        med.vs = _lib.fptr(keep[1]) if len(keep) == 3 else C.POINTER(C.c_float)()
+        return med, keep
+
+    def device_coef(self, name: str) -> np.ndarray:
+        """A coefficient array as held on the device (``gpi_model_get_coef``)."""
+        self._ensure_model()
+        from .fields import field_shape as fshape
+        grid = {"invK": "p", "invlambda": "tauxx", "invmu": "tauxx", "rho": "p" if self.physics == 0 else "tauxx"}
+        f = grid.get(name) or coef_grid(self.physics, self.ndim, name)
+        out = np.zeros(fshape(self.physics, self.ndim, self.order, self.ext_shape, f), F32, order="F")
+        self._ctx.check(self._ctx.lib.gpi_model_get_coef(self._ctx.h, self._model, COEF_ID[name], _lib.fptr(out),
+                                                          out.size))
+        return out
 
     def free(self):
         if self._model is not None and self._ctx is not None and self._ctx.h:
@@ -443,7 +502,7 @@ class SeisForwExpt:
         return tuple(out)
 
     def _upload_coefs(self):
-        self.coefs = dependent(self.mod, self.dt32, self.order, self.ndim)
+        self._coefs = dependent(self.mod, self.dt32, self.order, self.ndim)
         if self._model is not None:
             keep, arrs = self.coef_arrays()
             self._ctx.check(self._ctx.lib.gpi_model_update(self._ctx.h, self._model, arrs, len(arrs)))
@@ -478,7 +537,10 @@ class SeisForwExpt:
             raise ValueError("medium must match the experiment's medium type and grid")
         self.medium = medium.copy()
         self.exmedium = padarray(self.medium, self.npml, self.pml_faces)
-        self.mod = independent(self.exmedium)
+        self._mod = None
+        if self.coef_mode == "device" and self._coefs is None:
+            self.free()   # the next run derives the coefficients on the device again
+            return
         self._upload_coefs()
 
     def update_srcwav(self, srcwav, src_types=None):

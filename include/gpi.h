@@ -168,6 +168,27 @@ int gpi_model_create(gpi_ctx* ctx, const gpi_grid* grid, const gpi_array* coefs,
 int gpi_model_update(gpi_ctx* ctx, uint64_t model, const gpi_array* coefs, int32_t ncoefs);
 int gpi_model_free(gpi_ctx* ctx, uint64_t model);
 
+/* A medium on its unpadded grid (column-major, z fastest; vs for elastic only). */
+typedef struct gpi_medium {
+    int64_t n[3];        /* nz, 1, nx of the unpadded medium */
+    const float* vp;
+    const float* vs;
+    const float* rho;
+} gpi_medium;
+
+/* gpi_model_create with the coefficient arrays derived on the device: the medium side of
+ * update!(pac, medium) (src/fdtd/medium.jl:128-186) after padarray (src/media/media.jl:
+ * 260-304): replicate padding by npml on the PML faces, invK (or invlambda, invmu) and
+ * every dt-scaled array in the same FP32 expressions as the host preparation.  2-D only;
+ * grid->n must be the medium size plus the padding. */
+int gpi_model_create_medium(gpi_ctx* ctx, const gpi_grid* grid, const gpi_medium* medium,
+                            const gpi_pml* pml, int32_t npml_arrays, uint64_t* model_out);
+
+/* Copy a coefficient array of a model to the host (count = its field grid's size; the
+ * independent invK/invlambda/invmu live on the stress grid).  No reference counterpart:
+ * the reference reads pac.dmod directly. */
+int gpi_model_get_coef(gpi_ctx* ctx, uint64_t model, int32_t coef, float* out, int64_t count);
+
 /* ---- forward modelling ---------------------------------------------------------------- */
 /* One supersource of update!(pa) in :forward mode (src/fdtd/propagate.jl:74-135 with the
  * body of mod_x_proc!, :138-261): zero the wavefields and CPML memory, run nt steps
@@ -209,6 +230,62 @@ typedef struct gpi_adjoint_shot {
 int gpi_adjoint(gpi_ctx* ctx, uint64_t model, uint64_t store, const gpi_adjoint_shot* shot, float dtI,
                 float* g_invK, float* g_rho);
 int gpi_store_free(gpi_ctx* ctx, uint64_t store);
+
+/* ---- serving: continuous batching of many shots on one GPU ------------------------- */
+/* No reference counterpart (the reference runs supersources one after another per worker,
+ * src/fdtd/propagate.jl:100-106).  A server owns a device and a scheduler thread.  Shots
+ * (2-D, one physics and stencil order per server) are admitted as they arrive while
+ * HBM and slots allow, stay resident (medium, wavefields, CPML memory, sources,
+ * receivers) and advance together: every time step is one velocity and one stress launch
+ * covering all resident shots, each at its own step.  Finished shots leave and waiting
+ * ones enter between chunks of steps, so requests that arrive while others run overlap
+ * with them on the device.  Each shot's records equal those of gpi_forward on the same
+ * inputs bit for bit (same kernels, same order of operations). */
+typedef struct gpi_server gpi_server;
+
+typedef struct gpi_server_opts {
+    int32_t physics;             /* 0 acoustic, 1 elastic */
+    int32_t order;               /* 2, 4, 6, 8 */
+    int32_t max_shots;           /* resident shots at most (default 64) */
+    int32_t chunk_steps;         /* time steps between admissions (default 32) */
+    int32_t autotune;            /* 1: pick the workgroup shape per batch size by timing */
+    int32_t tile;                /* fixed shape index when autotune = 0 (0..3) */
+    uint64_t hbm_budget_bytes;   /* 0: 90 % of the free HBM at creation */
+} gpi_server_opts;
+
+/* One shot: the medium on its unpadded grid (coefficients are derived on the device as in
+ * gpi_model_create_medium), the CPML vectors, and a gpi_shot whose record buffers must
+ * stay valid until the shot is done.  Everything else is copied at submission. */
+typedef struct gpi_request {
+    gpi_grid grid;
+    gpi_medium medium;
+    const gpi_pml* pml;
+    int32_t npml_arrays;
+    int32_t reserved;
+    gpi_shot shot;               /* nfinal must be 0 */
+} gpi_request;
+
+typedef struct gpi_server_stats {
+    int64_t submitted, completed, failed;
+    int64_t active, queued;
+    int64_t steps;               /* batched time steps run */
+    int64_t launches;
+    double busy_ms;              /* device time of the batched steps */
+    double p50_ms, p99_ms;       /* latency (submission to records in the caller's buffers) */
+    double mean_batch;           /* mean resident shots per step */
+    int32_t tile_ty, tile_rx;    /* workgroup shape of the last chunk */
+    uint64_t hbm_used, hbm_budget;
+} gpi_server_stats;
+
+int gpi_server_create(int device, const gpi_server_opts* opts, gpi_server** out);
+/* Waits for every submitted shot, then stops the scheduler and frees the device state. */
+void gpi_server_destroy(gpi_server* srv);
+const char* gpi_server_error(const gpi_server* srv);
+int gpi_submit(gpi_server* srv, const gpi_request* req, uint64_t* ticket);
+/* status: 0 done, 1 pending, < 0 the shot's error code (message in gpi_server_error). */
+int gpi_poll(gpi_server* srv, uint64_t ticket, int32_t* status);
+int gpi_wait(gpi_server* srv, uint64_t ticket, double timeout_s, int32_t* status);
+int gpi_server_get_stats(gpi_server* srv, gpi_server_stats* out);
 
 /* Device timing of the last gpi_forward on this context. */
 int gpi_last_timing(const gpi_ctx* ctx, gpi_timing* out);
