@@ -81,7 +81,10 @@ struct Store {
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    bool async_alloc = false;            // allocate/free stream-ordered on `stream`
+    // When set, frees are deferred: pointers and bytes go here and the owner releases them
+    // once no queued work can reference them (the serving scheduler drains its streams first).
+    std::vector<void*>* defer_free = nullptr;
+    uint64_t* defer_bytes = nullptr;
     hipEvent_t ev[4] = {};
     std::vector<hipEvent_t> sample_ev;   // 4 events per sampled step
     uint64_t budget = 0, used = 0;

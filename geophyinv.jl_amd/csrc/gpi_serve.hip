@@ -5,8 +5,9 @@
 // receivers) while the compute stream runs the batch.  Every time step of the batch is
 // one velocity and one stress launch over all resident shots (fdtd2d.hip, batched
 // kernels), each shot at its own step.  Between chunks of steps, finished shots leave
-// (records copied to pinned memory, device memory released stream-ordered) and waiting
-// shots enter as slots and the HBM budget allow.  At most two chunks are in flight, so
+// (records copied to pinned memory) and waiting shots enter as slots and the HBM budget
+// allow.  Device memory comes from hipMalloc; the memory of departed shots is released
+// only after both streams have drained, so no queued launch can reference freed memory.  At most two chunks are in flight, so
 // admission and copy-back overlap the device work.  The workgroup shape of the batched
 // kernels is chosen per batch size by timing each candidate on live chunks.
 //
@@ -113,8 +114,10 @@ uint64_t estimate_bytes(const Req& r)
 
 struct gpi_server {
     gpi_server_opts o{};
-    gpi_ctx* cc = nullptr;   // compute stream, stream-ordered frees
-    gpi_ctx* ac = nullptr;   // admission stream, stream-ordered allocations
+    gpi_ctx* cc = nullptr;   // compute stream
+    gpi_ctx* ac = nullptr;   // admission stream
+    std::vector<void*> grave;     // device memory of departed shots, freed once drained
+    uint64_t grave_bytes = 0;
     std::thread th;
     std::mutex mu;
     std::condition_variable cv, cvd;
@@ -128,6 +131,7 @@ struct gpi_server {
     int64_t submitted = 0, completed = 0, failed = 0, steps = 0, launches = 0;
     double busy_ms = 0.0, batch_steps = 0.0;
     std::vector<double> latency;
+    std::map<uint64_t, double> lat_of;   // per finished ticket
     int last_ty = 4, last_rx = 1;
     uint64_t used = 0, budget = 0;
     int64_t nactive = 0;
@@ -144,12 +148,23 @@ struct gpi_server {
     std::map<int, int> best_shape; // batch-size class -> shape index
 
     void run();
+    void collect();
     int admit(std::unique_ptr<Req> r, Slot** out);
     void retire(Slot* s);
     void finalize(Slot* s, int code);
     void fail_req(uint64_t ticket, int code, const std::string& msg);
     int upload_table(const TileShape& t, int* nbv, int* nbs);
 };
+
+void gpi_server::collect()
+{
+    if (grave.empty()) return;
+    (void)hipStreamSynchronize(ac->stream);
+    (void)hipStreamSynchronize(cc->stream);
+    for (void* p : grave) (void)hipFree(p);
+    grave.clear();
+    grave_bytes = 0;
+}
 
 void gpi_server::fail_req(uint64_t ticket, int code, const std::string& msg)
 {
@@ -207,7 +222,7 @@ int gpi_server::admit(std::unique_ptr<Req> r, Slot** out)
 
 void gpi_server::retire(Slot* s)
 {
-    // records -> pinned staging, then release the device memory in stream order
+    // records -> pinned staging; the device memory goes to the graveyard
     size_t off = 0;
     for (const DevPoints& D : s->rec) {
         const size_t n = (size_t)s->r->nt * (size_t)std::max(D.npts, 1);
@@ -244,6 +259,7 @@ void gpi_server::finalize(Slot* s, int code)
             status[q.ticket] = 0;
             ++completed;
             latency.push_back(now_ms() - q.t_submit);
+            lat_of[q.ticket] = latency.back();
         } else {
             status[q.ticket] = code;
             ++failed;
@@ -267,6 +283,8 @@ int gpi_server::upload_table(const TileShape& t, int* nbv, int* nbs)
         BatchShot2D e{};
         if (!vel_params_2d(*s->m, *s->m->st0, &e.vp) || !str_params_2d(*s->m, *s->m->st0, &e.sp))
             return fail(cc, GPI_EINVAL, "internal error: incomplete wavefield state");
+        if (e.vp.nz != (int)s->m->g.n[0] || e.vp.ld != (int)s->m->pitch0 || !s->S)
+            return fail(cc, GPI_EINVAL, "internal error: inconsistent shot state");
         velocity_grid_2d(*s->m, t, &e.vp.gx, &e.vp.gy);
         stress_grid_2d(*s->m, t, &e.sp.gx, &e.sp.gy);
         e.xv = s->XV;
@@ -327,6 +345,8 @@ void gpi_server::run()
     for (;;) {
         // ---- admission ----------------------------------------------------------------
         std::vector<std::unique_ptr<Req>> take;
+        bool blocked = false;
+        if (active.empty() && finishing.empty()) collect();   // idle: release departed shots
         {
             std::unique_lock<std::mutex> lk(mu);
             if (active.empty() && finishing.empty() && queue.empty()) {
@@ -336,12 +356,17 @@ void gpi_server::run()
             }
             while (!queue.empty() && (int)(active.size() + take.size()) < o.max_shots) {
                 const uint64_t need = queue.front()->est;
-                if (used + need > budget && !(active.empty() && take.empty() && used == 0)) break;
+                if (used + grave_bytes + need > budget && !(active.empty() && take.empty() && used == 0)) {
+                    blocked = true;
+                    break;
+                }
                 used += need;
                 take.push_back(std::move(queue.front()));
                 queue.pop_front();
             }
         }
+        // memory of departed shots holds back admission: drain and release it
+        if ((blocked && grave_bytes > 0) || grave_bytes > budget / 8) collect();
         bool admitted = false;
         for (auto& r : take) {
             const uint64_t ticket = r->ticket;
@@ -477,6 +502,7 @@ void gpi_server::run()
     (void)hipStreamSynchronize(cc->stream);
     for (int k = 0; k < 2; ++k)
         if (pend[k] > 0) account(ck_ev[k][0], ck_ev[k][1], pend[k]);
+    collect();
 }
 
 extern "C" {
@@ -500,8 +526,10 @@ int gpi_server_create(int device, const gpi_server_opts* opts, gpi_server** out)
         delete v;
         return rc;
     }
-    v->cc->async_alloc = true;
-    v->ac->async_alloc = true;
+    v->cc->defer_free = &v->grave;
+    v->cc->defer_bytes = &v->grave_bytes;
+    v->ac->defer_free = &v->grave;
+    v->ac->defer_bytes = &v->grave_bytes;
     v->cc->budget = 0;
     v->ac->budget = 0;
     size_t fr = 0, tot = 0;
@@ -672,6 +700,16 @@ int gpi_wait(gpi_server* v, uint64_t ticket, double timeout_s, int32_t* st)
     if (timeout_s > 0) v->cvd.wait_for(lk, std::chrono::duration<double>(timeout_s), pred);
     else v->cvd.wait(lk, pred);
     *st = v->status[ticket];
+    return GPI_OK;
+}
+
+int gpi_latency(gpi_server* v, uint64_t ticket, double* ms)
+{
+    if (!v || !ms) return GPI_EINVAL;
+    std::lock_guard<std::mutex> lk(v->mu);
+    auto it = v->lat_of.find(ticket);
+    if (it == v->lat_of.end()) return GPI_ENOTFOUND;
+    *ms = it->second;
     return GPI_OK;
 }
 

@@ -85,8 +85,7 @@ int create_model_medium(gpi_ctx* c, const gpi_grid* g, const gpi_medium* med, co
 int prep_coefficients_2d(gpi_ctx* c, Model& m, const float* vp, const float* vs, const float* rho,
                          const int64_t nm[3], hipStream_t s);
 
-// host -> device copy that also works on contexts with stream-ordered allocation (the
-// allocation must have completed before a blocking copy touches it)
+// blocking host -> device copy
 int copy_h2d(gpi_ctx* c, void* dst, const void* src, size_t bytes);
 
 int upload_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3]);
@@ -101,7 +100,7 @@ struct Scratch {
     ~Scratch()
     {
         if (p.empty()) return;
-        if (!c->async_alloc) (void)hipStreamSynchronize(c->stream);
+        if (!c->defer_free) (void)hipStreamSynchronize(c->stream);
         free_allocs(c, p, bytes);
     }
     template <typename T>

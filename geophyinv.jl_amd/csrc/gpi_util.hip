@@ -113,8 +113,7 @@ int dev_alloc(gpi_ctx* c, std::vector<void*>* owner, uint64_t* owner_bytes, void
 {
     if (c->budget && c->used + bytes > c->budget)
         return fail(c, GPI_ENOMEM, "allocation of " + std::to_string(bytes) + " bytes exceeds the context's HBM budget");
-    // stream-ordered allocation on contexts that run a scheduler (no device-wide sync)
-    hipError_t e = c->async_alloc ? hipMallocAsync(p, bytes, c->stream) : hipMalloc(p, bytes);
+    hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) return fail(c, GPI_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     c->used += bytes;
     owner->push_back(*p);
@@ -124,7 +123,12 @@ int dev_alloc(gpi_ctx* c, std::vector<void*>* owner, uint64_t* owner_bytes, void
 
 void free_allocs(gpi_ctx* c, std::vector<void*>& allocs, uint64_t& bytes)
 {
-    for (void* p : allocs) (void)(c->async_alloc ? hipFreeAsync(p, c->stream) : hipFree(p));
+    if (c->defer_free) {
+        c->defer_free->insert(c->defer_free->end(), allocs.begin(), allocs.end());
+        if (c->defer_bytes) *c->defer_bytes += bytes;
+    } else {
+        for (void* p : allocs) (void)hipFree(p);
+    }
     allocs.clear();
     c->used -= std::min<uint64_t>(c->used, bytes);
     bytes = 0;
@@ -181,7 +185,6 @@ int zero_state(gpi_ctx* c, const Model& m, const State& st, hipStream_t s)
 
 int copy_h2d(gpi_ctx* c, void* dst, const void* src, size_t bytes)
 {
-    if (c->async_alloc) HIPTRY(c, hipStreamSynchronize(c->stream));
     HIPTRY(c, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return GPI_OK;
 }

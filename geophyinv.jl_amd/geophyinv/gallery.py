@@ -78,3 +78,53 @@ def elastic_layers_2d(n: int = 2001, ds: float = 2.5, nt: int = 4000, npml: int 
     src.update(["tauxx"], ricker(freq, Grid(0.0, dt, nw))[:nt])
     kw = dict(rfields=["vx", "vz"], npml=npml)
     return medium, ageom, [src], tgrid, kw
+
+
+def serving_stream(nshots: int = 512, seed: int = 3003, rate: float = 200.0, ds: float = 5.0,
+                   side=(256, 2048), steps=(1000, 6000), freq=(10.0, 30.0), receivers=(64, 512),
+                   npml: int = 20):
+    """A stream of 2-D acoustic shots arriving as a Poisson process.
+
+    Each shot: square grid with side ~ U{side}, dx = ds; nt ~ U{steps}; Ricker with peak
+    ~ U(freq) on p five cells below the surface at a random x; nr ~ U{receivers} evenly
+    spaced along the surface recording p and vz; a layered medium (3-10 layers, vp
+    increasing within [1500, 4500] m/s, Gardner density) with one Gaussian anomaly of
+    +-10 % and radius ~ U(50, 250) m; CFL 0.5; npml CPML cells on every face.
+
+    Returns a list of dicts with ``arrival`` (seconds from the first arrival) and the
+    inputs of ``SeisForwExpt`` (``medium``, ``ageom``, ``srcwav``, ``tgrid``, ``kw``)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    gaps = rng.exponential(1.0 / rate, nshots)
+    gaps[0] = 0.0
+    arrivals = np.cumsum(gaps)
+    out = []
+    for k in range(nshots):
+        s = int(rng.integers(side[0], side[1] + 1))
+        nt = int(rng.integers(steps[0], steps[1] + 1))
+        f = float(rng.uniform(*freq))
+        nr = int(rng.integers(receivers[0], receivers[1] + 1))
+        nl = int(rng.integers(3, 11))
+        L = (s - 1) * ds
+        grid = [Grid.range(0.0, L, length=s), Grid.range(0.0, L, length=s)]
+        tops = np.sort(rng.uniform(0.0, L, nl - 1))
+        vl = np.sort(rng.uniform(1500.0, 4500.0, nl))
+        z = grid[0].values()
+        vcol = vl[np.searchsorted(tops, z, side="right")]
+        amp = float(rng.choice([-0.10, 0.10]))
+        rad = float(rng.uniform(50.0, 250.0))
+        cz, cx = rng.uniform(0.2 * L, 0.8 * L, 2)
+        zz = z[:, None]
+        xx = grid[1].values()[None, :]
+        vp = np.asfortranarray(vcol[:, None] * (1.0 + amp * np.exp(-((zz - cz) ** 2 + (xx - cx) ** 2)
+                                                                   / (2.0 * rad ** 2))), dtype=np.float32)
+        rho = np.asfortranarray(310.0 * vp.astype(np.float64) ** 0.25, dtype=np.float32)
+        medium = AcousticMedium(grid, vp, rho)
+        dt = 0.5 * ds / float(vp.max())
+        tgrid = Grid(0.0, dt, nt)
+        ageom = [AGeomss({"z": [5 * ds], "x": [float(rng.uniform(0.1 * L, 0.9 * L))]},
+                         {"z": np.zeros(nr), "x": np.linspace(0.0, L, nr)})]
+        src = Srcs(1, tgrid, ["p"])
+        src.update(["p"], ricker(f, tgrid))
+        out.append(dict(arrival=float(arrivals[k]), medium=medium, ageom=ageom, srcwav=[src], tgrid=tgrid,
+                        kw=dict(rfields=["p", "vz"], npml=npml), side=s, nt=nt))
+    return out

@@ -37,7 +37,11 @@ class MediumParameter:
         self.bounds = [F32(bounds[0]), F32(bounds[1])]
 
     def copy(self) -> "MediumParameter":
-        return MediumParameter(self.m.copy(), self.ref, list(self.bounds))
+        out = MediumParameter.__new__(MediumParameter)   # validated already: copy as is
+        out.m = self.m.copy(order="K")
+        out.ref = self.ref
+        out.bounds = list(self.bounds)
+        return out
 
 
 class Medium:

@@ -75,11 +75,45 @@ def point_weights(mgrid_f: list, pos, compat: int = 0):
     return np.array(idx, np.int64), np.array(w, np.float64).astype(np.float32)
 
 
+def _axis(g: Grid, v: np.ndarray):
+    """Vectorised ``neighbours`` and ``_fraction`` for many coordinates on one axis."""
+    vals = g.values()
+    n = g.n
+    idx = np.searchsorted(vals, v, side="left")
+    lo = np.where(idx == 0, 0, np.where(idx >= n - 1, n - 2, idx - 1))
+    hi = lo + 1
+    x1, x2 = vals[lo], vals[hi]
+    f = np.where(v < g.first, 0.0, np.where(v > g.last, 1.0, (v - x1) / (x2 - x1)))
+    return lo, hi, f
+
+
 def get_proj_matrix(points, mgrid_f: list, compat: int = 0):
-    """All columns for a list of points: ``idx`` (npts, nnz) int64 and ``w`` (npts, nnz) f32."""
-    nnz = 4 if len(mgrid_f) == 2 else 8
-    idx = np.zeros((len(points), nnz), np.int64)
-    w = np.zeros((len(points), nnz), np.float32)
-    for k, p in enumerate(points):
-        idx[k], w[k] = point_weights(mgrid_f, p, compat)
-    return idx, w
+    """All columns for a list of points: ``idx`` (npts, nnz) int64 and ``w`` (npts, nnz) f32.
+
+    The same arithmetic as ``point_weights`` (FP64 products rounded once to FP32), for all
+    points at once."""
+    pts = np.asarray(points, np.float64).reshape(len(points), len(mgrid_f))
+    if len(mgrid_f) == 2:
+        gz, gx = mgrid_f
+        z1, z2, dz = _axis(gz, pts[:, 0])
+        x1, x2, dx = _axis(gx, pts[:, 1])
+        nz = gz.n
+        idx = np.stack([z1 + nz * x1, z2 + nz * x1, z1 + nz * x2, z2 + nz * x2], axis=1)
+        w = np.stack([(1 - dz) * (1 - dx), dz * (1 - dx), (1 - dz) * dx, dz * dx], axis=1)
+        return idx.astype(np.int64), w.astype(np.float32)
+    gz, gy, gx = mgrid_f
+    pz, py, px = pts[:, 0], pts[:, 1], pts[:, 2]
+    if compat & COMPAT_SWAP3D:
+        pz, px = px, pz
+    z, y, x = _axis(gz, pz), _axis(gy, py), _axis(gx, px)
+    nz, ny = gz.n, gy.n
+    idx, w = [], []
+    for bx in (0, 1):
+        for by in (0, 1):
+            for bz in (0, 1):
+                idx.append(z[bz] + nz * (y[by] + ny * x[bx]))
+                wz = z[2] if bz else 1 - z[2]
+                wy = y[2] if by else 1 - y[2]
+                wx = x[2] if bx else 1 - x[2]
+                w.append((wz * wy) * wx)
+    return np.stack(idx, axis=1).astype(np.int64), np.stack(w, axis=1).astype(np.float32)

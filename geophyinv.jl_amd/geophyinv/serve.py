@@ -99,6 +99,12 @@ class Server:
             raise _lib.GpiError(st.value, self.lib.gpi_server_error(self.h).decode())
         return self._out.pop(ticket)
 
+    def latency_ms(self, ticket: int) -> float:
+        """Submission to records-on-host time of a finished shot."""
+        ms = C.c_double()
+        self._check(self.lib.gpi_latency(self.h, ticket, C.byref(ms)))
+        return ms.value
+
     def stats(self) -> dict:
         s = _lib.gpi_server_stats()
         self._check(self.lib.gpi_server_get_stats(self.h, C.byref(s)))

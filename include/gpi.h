@@ -286,6 +286,9 @@ int gpi_submit(gpi_server* srv, const gpi_request* req, uint64_t* ticket);
 int gpi_poll(gpi_server* srv, uint64_t ticket, int32_t* status);
 int gpi_wait(gpi_server* srv, uint64_t ticket, double timeout_s, int32_t* status);
 int gpi_server_get_stats(gpi_server* srv, gpi_server_stats* out);
+/* Milliseconds from gpi_submit to the records being in the caller's buffers, for a
+ * finished shot (GPI_ENOTFOUND until then). */
+int gpi_latency(gpi_server* srv, uint64_t ticket, double* ms);
 
 /* Device timing of the last gpi_forward on this context. */
 int gpi_last_timing(const gpi_ctx* ctx, gpi_timing* out);

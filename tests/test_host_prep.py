@@ -82,3 +82,23 @@ def _experiment3d(physics, order, compat):
 @pytest.mark.parametrize("compat", [0, 1])
 def test_prep_3d(physics, order, compat):
     check_prep(_experiment3d(physics, order, compat))
+
+
+def test_vectorised_weights_equal_pointwise():
+    """get_proj_matrix (all points at once) equals point_weights (one point at a time), inside
+    and outside the grid, on nodes, 2-D and 3-D, both 3-D coordinate conventions."""
+    from geophyinv.grid import Grid
+    from geophyinv.projmat import get_proj_matrix, point_weights
+    rng = np.random.default_rng(0)
+    for nd in (2, 3):
+        g = ([Grid(-3.0, 2.5, 40), Grid(1.0, 5.0, 33)] if nd == 2
+             else [Grid(-3.0, 2.5, 20), Grid(0.0, 4.0, 17), Grid(1.0, 5.0, 23)])
+        lo = np.array([x.first for x in g]) - 10
+        hi = np.array([x.last for x in g]) + 10
+        pts = rng.uniform(lo, hi, (300, nd))
+        pts[:20] = np.array([x.values()[3] for x in g])
+        for compat in (0, 1):
+            I, W = get_proj_matrix(pts, g, compat)
+            for k, p in enumerate(pts):
+                i, w = point_weights(g, p, compat)
+                assert np.array_equal(I[k], i) and np.array_equal(W[k], w), (nd, compat, k)
