@@ -261,7 +261,7 @@ constexpr int PLAIN_RX = 1;
 template <int O, int PHYS>
 __global__ __launch_bounds__(64 * PLAIN_TY) void vel2d_kernel(VelParams2D P, PassExtras X)
 {
-    const int b = blockIdx.x;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int nbm = P.gx * P.gy;
     if (b >= nbm) {
         gather_block(X, X.it, b - nbm);
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(64 * PLAIN_TY) void vel2d_kernel(VelParams2D P, Pas
 template <int O, int PHYS>
 __global__ __launch_bounds__(64 * PLAIN_TY) void str2d_kernel(StrParams2D P, PassExtras X)
 {
-    const int b = blockIdx.x;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int nbm = P.gx * P.gy;
     if (b >= nbm) {
         gather_block(X, X.it, b - nbm);
@@ -326,7 +326,7 @@ template <int O, int PHYS, int TY, int RX>
 __global__ __launch_bounds__(64 * TY) void vel2d_batch(const BatchShot2D* __restrict__ D, const int* __restrict__ first,
                                                        int n, int64_t step)
 {
-    const int b = blockIdx.x;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int s = shot_of_block(first, n, b);
     const BatchShot2D& S = D[s];
     const int64_t it = step - S.start;
@@ -344,7 +344,7 @@ template <int O, int PHYS, int TY, int RX>
 __global__ __launch_bounds__(64 * TY) void str2d_batch(const BatchShot2D* __restrict__ D, const int* __restrict__ first,
                                                        int n, int64_t step)
 {
-    const int b = blockIdx.x;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int s = shot_of_block(first, n, b);
     const BatchShot2D& S = D[s];
     const int64_t it = step - S.start;

@@ -87,7 +87,7 @@ template <int O>
 __global__ __launch_bounds__(256) void vel3d_acoustic(VelParams3D P, PassExtras X)
 {
     constexpr int H = O - 1;
-    const int b = blockIdx.x;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int nxw = P.nx - H;
     if (b >= P.gx * P.gy * nxw) {
         gather_block(X, b - P.gx * P.gy * nxw);
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void vel3d_acoustic(VelParams3D P, PassExtras 
 template <int O>
 __global__ __launch_bounds__(256) void str3d_acoustic(StrParams3D P, PassExtras X)
 {
-    const int b = blockIdx.x;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= P.gx * P.gy * P.nx) {
         gather_block(X, b - P.gx * P.gy * P.nx);
         return;
@@ -173,7 +173,7 @@ template <int O>
 __global__ __launch_bounds__(256) void vel3d_elastic(VelParams3D P, PassExtras X)
 {
     constexpr int H = O - 1;
-    const int b = blockIdx.x;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int nxw = P.nx - H;
     if (b >= P.gx * P.gy * nxw) {
         gather_block(X, b - P.gx * P.gy * nxw);
@@ -237,7 +237,7 @@ template <int O>
 __global__ __launch_bounds__(256) void str3d_elastic(StrParams3D P, PassExtras X)
 {
     constexpr int H = O - 1;
-    const int b = blockIdx.x;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= P.gx * P.gy * P.nx) {
         gather_block(X, b - P.gx * P.gy * P.nx);
         return;

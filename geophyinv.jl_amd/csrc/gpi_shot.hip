@@ -379,6 +379,19 @@ int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
         if (f < 0 || f >= GPI_NFIELDS || !is_wavefield(f) || !st.field[f] || !shot->final_out[q])
             return fail(c, GPI_EINVAL, "bad final field request");
     }
+    // snapshots: device copies of the field after the requested steps (snaps_save!,
+    // src/fdtd/fdtd.jl:587-591, called at propagate.jl:237-244)
+    std::vector<float*> snap;
+    if (shot->nsnap > 0) {
+        const int f = shot->snap_field;
+        if (f < 0 || f >= GPI_NFIELDS || !is_wavefield(f) || !st.field[f] || !shot->snap_steps || !shot->snap_out)
+            return fail(c, GPI_EINVAL, "bad snapshot request");
+        snap.resize(shot->nsnap);
+        for (int q = 0; q < shot->nsnap; ++q) {
+            if (!shot->snap_out[q]) return fail(c, GPI_EINVAL, "null snapshot output");
+            if ((rc = S.alloc(&snap[q], (size_t)m.slab))) return rc;
+        }
+    }
 
     // reset wavefields and CPML memory (reset_w2!, types.jl:101-113)
     if ((rc = zero_state(c, m, st, c->stream))) return rc;
@@ -402,6 +415,10 @@ int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
         for (int q : src.sfall) inject(m, st, src.pts[q], it, c->stream);
         launches += 2 + (int64_t)src.vfall.size() + (int64_t)src.sfall.size();
         if (save) boundary_strips(m, st, *save, it, true, c->stream, &launches);
+        for (int q = 0; q < shot->nsnap; ++q)
+            if (shot->snap_steps[q] == it)
+                HIPTRY(c, hipMemcpyAsync(snap[q], st.field[shot->snap_field], sizeof(float) * (size_t)m.slab,
+                                         hipMemcpyDeviceToDevice, c->stream));
         smp.done(it);
     }
     if (save)   // boundary_save_snap_tau!: -stress at nt (boundary.jl:268-287)
@@ -433,6 +450,11 @@ int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
         shape_of(g.ndim, g.order, g.physics, g.n, f, n3);
         if ((rc = download_pitched(c, m, shot->final_out[q], st.field[f], n3))) return rc;
     }
+    for (int q = 0; q < shot->nsnap; ++q) {
+        int64_t n3[3];
+        shape_of(g.ndim, g.order, g.physics, g.n, shot->snap_field, n3);
+        if ((rc = download_pitched(c, m, shot->snap_out[q], snap[q], n3))) return rc;
+    }
     return GPI_OK;
 }
 
@@ -444,7 +466,7 @@ int check_shot(gpi_ctx* c, uint64_t model, const gpi_shot* shot, Model** out)
     if (shot->nt < 1) return fail(c, GPI_EINVAL, "nt must be positive");
     if (shot->nsrc < 0 || shot->nrec < 0 || shot->nsrc > 8 || shot->nrec > 16 || shot->nfinal < 0 ||
         (shot->nsrc > 0 && !shot->src) || (shot->nrec > 0 && !shot->rec) ||
-        (shot->nfinal > 0 && (!shot->final_fields || !shot->final_out)))
+        (shot->nfinal > 0 && (!shot->final_fields || !shot->final_out)) || shot->nsnap < 0)
         return fail(c, GPI_EINVAL, "bad number of source/receiver fields");
     HIPTRY(c, hipSetDevice(c->device));
     *out = m;

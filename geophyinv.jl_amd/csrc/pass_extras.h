@@ -20,6 +20,17 @@ namespace gpi {
 
 enum InjKind { INJ_P = 0, INJ_NORMAL2 = 1, INJ_NORMAL3 = 2, INJ_VX = 3, INJ_VY = 4, INJ_VZ = 5 };
 
+// Workgroup order for the 8 XCDs of MI355X: workgroups are dispatched to XCDs round
+// robin (workgroup w to XCD w % 8), so tile t = remap(w) gives each XCD one contiguous
+// range of tiles.  Neighbouring tiles, which read each other's halo columns or planes,
+// then share the XCD's L2.  A bijection of [0, n).
+__device__ __forceinline__ int xcd_remap(int w, int n)
+{
+    const int q = n >> 3, r = n & 7;
+    const int k = w & 7, i = w >> 3;
+    return k * q + min(k, r) + i;
+}
+
 struct InjSet {
     int kind;
     int npts;              // sources of the field (wavelet row length)

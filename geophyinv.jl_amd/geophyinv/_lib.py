@@ -49,9 +49,12 @@ class gpi_shot(C.Structure):
     _fields_ = [("nt", C.c_int64), ("nsrc", C.c_int32), ("nrec", C.c_int32),
                 ("src", C.POINTER(gpi_points)), ("wavelets", C.POINTER(C.POINTER(C.c_float))),
                 ("rec", C.POINTER(gpi_points)), ("records", C.POINTER(C.POINTER(C.c_float))),
-                ("nfinal", C.c_int32), ("reserved", C.c_int32),
+                ("nfinal", C.c_int32), ("nsnap", C.c_int32),
                 ("final_fields", C.POINTER(C.c_int32)),
-                ("final_out", C.POINTER(C.POINTER(C.c_float)))]
+                ("final_out", C.POINTER(C.POINTER(C.c_float))),
+                ("snap_field", C.c_int32), ("reserved", C.c_int32),
+                ("snap_steps", C.POINTER(C.c_int64)),
+                ("snap_out", C.POINTER(C.POINTER(C.c_float)))]
 
 
 class gpi_adjoint_shot(C.Structure):

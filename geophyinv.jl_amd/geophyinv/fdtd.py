@@ -109,7 +109,8 @@ class SeisForwExpt:
     def __init__(self, attrib_mod: _Attrib, *, medium: Medium, tgrid: Grid, ageom, srcwav,
                  rfields=("vz",), pml_faces=None, rigid_faces=None, stressfree_faces=(),
                  order: int = 2, npml: int | None = None, compat: int = 1, verbose: bool = False,
-                 context: "_lib.Context | None" = None, coefficients: str = "host"):
+                 context: "_lib.Context | None" = None, coefficients: str = "host", tsnaps=None,
+                 snaps_field=None):
         ndim = medium.ndim
         if order not in (2, 4, 6, 8):
             raise ValueError("order must be 2, 4, 6 or 8")
@@ -156,6 +157,16 @@ class SeisForwExpt:
         self.stressfree_faces = list(stressfree_faces)
         self.rfields = rfields
         self.verbose = verbose
+        # snapshots (src/fdtd/fdtd.jl:74-75,179-188): the step nearest to each time in tsnaps
+        if snaps_field is not None and (not has_field(physics, ndim, snaps_field) or snaps_field.startswith("d")):
+            raise ValueError(f"invalid snaps field {snaps_field!r}")
+        self.snaps_field = snaps_field
+        if tsnaps is None:
+            tsnaps = [0.5 * (tgrid.values()[-1] + tgrid.values()[0])]
+        self.tsnaps = [float(t) for t in np.atleast_1d(tsnaps)]
+        tv = tgrid.values()
+        self.itsnaps = [int(np.argmin(np.abs(tv - t))) for t in self.tsnaps]
+        self.snaps = [None] * len(ageom)
         self.medium = medium.copy()
         self.ageom = ageom
         self.srcwav = [s.copy() for s in srcwav]
@@ -391,7 +402,7 @@ class SeisForwExpt:
         return ((_lib.gpi_points * max(1, len(pts)))(*pts), (C.POINTER(C.c_float) * max(1, len(wavs)))(*wavs),
                 len(pts))
 
-    def run_shot(self, iss: int, final_fields=(), save: bool = False):
+    def run_shot(self, iss: int, final_fields=(), save: bool = False, snaps: bool = True):
         """Propagate supersource ``iss`` on the GPU; returns (records dict, finals dict).
 
         ``save`` runs the :forward_save variant, which keeps the boundary strips and the
@@ -431,6 +442,17 @@ class SeisForwExpt:
         shot.nfinal = len(final_fields)
         shot.final_fields = fids.ctypes.data_as(C.POINTER(C.c_int32))
         shot.final_out = (C.POINTER(C.c_float) * max(1, len(fptrs)))(*fptrs)
+        snap_out = []
+        if snaps and self.snaps_field is not None:
+            shp = field_shape(self.physics, self.ndim, self.order, self.ext_shape, self.snaps_field)
+            snap_out = [np.zeros(shp, F32, order="F") for _ in self.itsnaps]
+            steps = np.array(self.itsnaps, np.int64)
+            keep.append(steps)
+            shot.nsnap = len(snap_out)
+            shot.snap_field = FIELD_ID[self.snaps_field]
+            shot.snap_steps = steps.ctypes.data_as(C.POINTER(C.c_int64))
+            shot.snap_out = (C.POINTER(C.c_float) * len(snap_out))(*[_lib.fptr(a) for a in snap_out])
+            self.snaps[iss] = snap_out
         if save:
             self._free_store(iss)
             h = C.c_uint64()
@@ -581,6 +603,8 @@ class SeisForwExpt:
             iss = None
         if key == "data":
             return self.data if iss is None else self.data[iss]
+        if key == "snaps":   # pa[:snaps, i] (src/fdtd/getprop.jl:13-24)
+            return self.snaps[0 if iss is None else iss]
         if key in ("ageom", "srcwav", "medium", "exmedium", "rfields", "pml_faces", "rigid_faces", "gradients",
                    "mod", "ref_mod"):
             return getattr(self, key)

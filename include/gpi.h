@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define GPI_ABI_VERSION 1
+#define GPI_ABI_VERSION 2
 
 /* error codes */
 #define GPI_OK 0
@@ -126,9 +126,13 @@ typedef struct gpi_shot {
     const gpi_points* rec;           /* nrec interpolation matrices (rinterpolatew) */
     float* const* records;           /* nrec host outputs of nt x npts, time fastest (Recs.d) */
     int32_t nfinal;                  /* optional final wavefields to copy back */
-    int32_t reserved;
+    int32_t nsnap;                   /* snapshots (snaps_field at tsnaps, fdtd.jl:179-188) */
     const int32_t* final_fields;     /* field ids */
     float* const* final_out;         /* host outputs, each of its field's full size */
+    int32_t snap_field;              /* field id of the snapshots */
+    int32_t reserved;
+    const int64_t* snap_steps;       /* nsnap zero-based steps: the field after that step */
+    float* const* snap_out;          /* nsnap host outputs of the field's full size */
 } gpi_shot;
 
 /* Timing of the last gpi_forward call on a context (device time, HIP events on the
@@ -262,7 +266,7 @@ typedef struct gpi_request {
     const gpi_pml* pml;
     int32_t npml_arrays;
     int32_t reserved;
-    gpi_shot shot;               /* nfinal must be 0 */
+    gpi_shot shot;               /* nfinal and nsnap must be 0 */
 } gpi_request;
 
 typedef struct gpi_server_stats {

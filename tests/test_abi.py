@@ -44,7 +44,9 @@ def test_library_exports_every_declared_symbol(lib):
 
 
 def test_abi_version(lib):
-    assert lib.gpi_abi_version() == 1
+    import re
+    hdr = open(os.path.join(ROOT, "include", "gpi.h")).read()
+    assert lib.gpi_abi_version() == int(re.search(r"#define GPI_ABI_VERSION (\d+)", hdr).group(1))
 
 
 @pytest.mark.parametrize("physics", [0, 1])

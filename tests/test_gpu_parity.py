@@ -153,3 +153,25 @@ def test_acoustic_3d_large_grid_short(gpu_ctx):
     pa = experiment3d(medium3d(0, n=(160, 160, 160), ds=10.0), nt=12, npml=5, ns=1, nr=40,
                       ctx=gpu_ctx, seed=3)
     compare(pa, ["p", "vz"], "acoustic 3-D 170^3")
+
+
+@pytest.mark.parametrize("physics,field", [(0, "p"), (0, "vz"), (1, "tauxz")])
+def test_snapshots(gpu_ctx, physics, field):
+    """pa[:snaps, i]: the field at the steps nearest to tsnaps equals the oracle's state
+    after that many steps (snaps_save! at the end of the step, propagate.jl:237-244)."""
+    med = layered_acoustic(nz=50, nx=58) if physics == 0 else layered_elastic(nz=50, nx=58)
+    base = experiment(med, nt=200, sfields=("p",) if physics == 0 else ("tauxx",),
+                      rfields=("vz",), ctx=gpu_ctx, seed=4)
+    tv = base.tgrid.values()
+    tsnaps = [tv[37] + 0.2 * base.tgrid.step, tv[120], tv[-1] + 1.0]
+    pa = G.SeisForwExpt(base.attrib_mod, medium=med, tgrid=base.tgrid, ageom=base.ageom, srcwav=base.srcwav,
+                        rfields=base.rfields, npml=base.npml, context=gpu_ctx, tsnaps=tsnaps, snaps_field=field)
+    assert pa.itsnaps == [37, 120, 199]
+    pa.update()
+    snaps = pa["snaps", 0]
+    for k, it in enumerate(pa.itsnaps):
+        orc = O.from_experiment(pa)
+        orc.run(it + 1)
+        ref = orc.field(field)
+        assert np.abs(ref).max() > 0
+        assert rel_l2(snaps[k], ref) <= TOL, (field, it)
