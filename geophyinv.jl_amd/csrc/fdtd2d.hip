@@ -15,6 +15,8 @@
 // Layout: every wavefield and coefficient array of a model uses the same column pitch
 // `ld` (floats), z contiguous.  A thread owns one derivative-grid point (i, j); lanes of a
 // wavefront run along z so every load and store is a coalesced 256-byte row segment.
+#include <cstdlib>
+
 #include "fdtd2d.h"
 #include "pass_extras.h"
 #include "stencil.h"
@@ -255,11 +257,21 @@ __device__ __forceinline__ void str_tile(const StrParams2D& P, const PassExtras&
 // One shot per launch (the drop-in path): 64 x 4 workgroups, one column per thread.
 // ---------------------------------------------------------------------------------------
 
-constexpr int PLAIN_TY = 4;
-constexpr int PLAIN_RX = 1;
+// Workgroup shape of the single-shot kernels: index into TILE_SHAPES, from GPI_TILE2D
+// (default 0: 64 x 4 threads, one column each).
+int plain_shape()
+{
+    static int t = -1;
+    if (t < 0) {
+        const char* e = getenv("GPI_TILE2D");
+        t = e ? atoi(e) : 0;
+        if (t < 0 || t > 3) t = 0;
+    }
+    return t;
+}
 
-template <int O, int PHYS>
-__global__ __launch_bounds__(64 * PLAIN_TY) void vel2d_kernel(VelParams2D P, PassExtras X)
+template <int O, int PHYS, int TY, int RX>
+__global__ __launch_bounds__(64 * TY) void vel2d_kernel(VelParams2D P, PassExtras X)
 {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int nbm = P.gx * P.gy;
@@ -267,40 +279,61 @@ __global__ __launch_bounds__(64 * PLAIN_TY) void vel2d_kernel(VelParams2D P, Pas
         gather_block(X, X.it, b - nbm);
         return;
     }
-    vel_tile<O, PHYS, PLAIN_TY, PLAIN_RX>(P, X, X.it, X.ninj, b);
+    vel_tile<O, PHYS, TY, RX>(P, X, X.it, X.ninj, b);
+}
+
+template <int O, int PHYS, int TY, int RX>
+__global__ __launch_bounds__(64 * TY) void str2d_kernel(StrParams2D P, PassExtras X)
+{
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int nbm = P.gx * P.gy;
+    if (b >= nbm) {
+        gather_block(X, X.it, b - nbm);
+        return;
+    }
+    str_tile<O, PHYS, TY, RX>(P, X, X.it, X.ninj, b);
+}
+
+template <int O, int PHYS, int TY, int RX>
+void launch_plain(bool velocity, hipStream_t s, VelParams2D V, StrParams2D S, const PassExtras& X)
+{
+    constexpr int H = O - 1;
+    if (velocity) {
+        V.gx = (V.nz - H + 63) / 64;
+        V.gy = (V.nx - H + TY * RX - 1) / (TY * RX);
+        hipLaunchKernelGGL((vel2d_kernel<O, PHYS, TY, RX>), dim3(V.gx * V.gy + X.gather_blocks), dim3(64, TY), 0, s,
+                           V, X);
+    } else {
+        S.gx = (S.nz + 63) / 64;
+        S.gy = (S.nx + TY * RX - 1) / (TY * RX);
+        hipLaunchKernelGGL((str2d_kernel<O, PHYS, TY, RX>), dim3(S.gx * S.gy + X.gather_blocks), dim3(64, TY), 0, s,
+                           S, X);
+    }
 }
 
 template <int O, int PHYS>
-__global__ __launch_bounds__(64 * PLAIN_TY) void str2d_kernel(StrParams2D P, PassExtras X)
+void launch_plain_o(bool velocity, hipStream_t s, const VelParams2D& V, const StrParams2D& S, const PassExtras& X)
 {
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const int nbm = P.gx * P.gy;
-    if (b >= nbm) {
-        gather_block(X, X.it, b - nbm);
-        return;
+    switch (plain_shape()) {
+    case 1: launch_plain<O, PHYS, 8, 1>(velocity, s, V, S, X); break;
+    case 2: launch_plain<O, PHYS, 2, 2>(velocity, s, V, S, X); break;
+    case 3: launch_plain<O, PHYS, 4, 2>(velocity, s, V, S, X); break;
+    default: launch_plain<O, PHYS, 4, 1>(velocity, s, V, S, X); break;
     }
-    str_tile<O, PHYS, PLAIN_TY, PLAIN_RX>(P, X, X.it, X.ninj, b);
 }
 
 template <int O>
 void launch_vel(const Model& m, hipStream_t s, VelParams2D P, const PassExtras& X)
 {
-    constexpr int H = O - 1;
-    P.gx = (P.nz - H + 63) / 64;
-    P.gy = (P.nx - H + PLAIN_TY * PLAIN_RX - 1) / (PLAIN_TY * PLAIN_RX);
-    dim3 grd(P.gx * P.gy + X.gather_blocks), blk(64, PLAIN_TY);
-    if (m.g.physics == 0) hipLaunchKernelGGL((vel2d_kernel<O, 0>), grd, blk, 0, s, P, X);
-    else hipLaunchKernelGGL((vel2d_kernel<O, 1>), grd, blk, 0, s, P, X);
+    if (m.g.physics == 0) launch_plain_o<O, 0>(true, s, P, StrParams2D{}, X);
+    else launch_plain_o<O, 1>(true, s, P, StrParams2D{}, X);
 }
 
 template <int O>
 void launch_str(const Model& m, hipStream_t s, StrParams2D P, const PassExtras& X)
 {
-    P.gx = (P.nz + 63) / 64;
-    P.gy = (P.nx + PLAIN_TY * PLAIN_RX - 1) / (PLAIN_TY * PLAIN_RX);
-    dim3 grd(P.gx * P.gy + X.gather_blocks), blk(64, PLAIN_TY);
-    if (m.g.physics == 0) hipLaunchKernelGGL((str2d_kernel<O, 0>), grd, blk, 0, s, P, X);
-    else hipLaunchKernelGGL((str2d_kernel<O, 1>), grd, blk, 0, s, P, X);
+    if (m.g.physics == 0) launch_plain_o<O, 0>(false, s, VelParams2D{}, P, X);
+    else launch_plain_o<O, 1>(false, s, VelParams2D{}, P, X);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -23,6 +23,8 @@ struct VelParams3D {
     int nz, ny, nx, npml;
     int64_t s1, s2;    // y and x strides of the pitched layout
     int gx, gy;        // workgroups along z and y; the grid's third factor runs over x
+    int kc;            // x planes per workgroup (acoustic passes march along x)
+    int remap;         // XCD-aware workgroup order
     uint32_t pml_mask, rigid_mask;
     float dxI, dyI, dzI, dt;
 };
@@ -41,6 +43,8 @@ struct StrParams3D {
     int nz, ny, nx, npml;
     int64_t s1, s2;
     int gx, gy;
+    int kc;            // x planes per workgroup (acoustic passes march along x)
+    int remap;         // XCD-aware workgroup order
     uint32_t pml_mask, free_mask;
     float dxI, dyI, dzI;
 };
