@@ -28,6 +28,15 @@ def strip_cells(n_cells, n_axis, npml, faces_on_axis):
     return n_cells * (npml * faces_on_axis) / n_axis
 
 
+def algorithmic_bytes_3d(n, npml):
+    """3-D acoustic: velocity pass reads p, vx, vy, vz and three density coefficients and writes
+    vx, vy, vz (40 B per cell); stress pass reads vx, vy, vz, p, dtK and writes p (24 B per
+    cell); one CPML memory variable read and written per strip cell per axis (8 B)."""
+    N = n[0] * n[1] * n[2]
+    strips = sum(strip_cells(N, n[a], npml, 2) for a in range(3))
+    return 40 * N + 8 * strips, 24 * N + 8 * strips
+
+
 def algorithmic_bytes(nz, nx, npml, physics):
     """Compulsory FP32 bytes of one velocity pass and one stress pass (each array read once,
     each output written once; derivatives never stored; CPML memory read+written in strips)."""
@@ -45,12 +54,25 @@ def algorithmic_bytes(nz, nx, npml, physics):
     return vel, stress
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed counter summary (FETCH_SIZE +
+    WRITE_SIZE runs, calibrated with a copy of known size), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "fdtd2d_elastic_2081x2081_traffic.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch %s (\d+)$" % re.escape(kernel), open(path).read(), re.M)
+    return int(m.group(1)) if m else None
+
+
 def cpu_baseline(pa, seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
+    import numpy as np
+
     orc = O.from_experiment(pa)
-    nz, nx = pa.ext_shape
+    cells = int(np.prod(pa.ext_shape))
     t0 = time.perf_counter()
     orc.run(8)
     probe = time.perf_counter() - t0
@@ -60,9 +82,9 @@ def cpu_baseline(pa, seconds):
     t0 = time.perf_counter()
     orc2.run(steps)
     dt = time.perf_counter() - t0
-    return {"value": nz * nx * steps / dt / 1e9, "unit": "Gpt-updates/s", "cores": O.threads(),
+    return {"value": cells * steps / dt / 1e9, "unit": "Gpt-updates/s", "cores": O.threads(),
             "kind": "port",
-            "sample": f"oracle (unfused C restatement, OpenMP) on the full {nz}x{nx} grid, "
+            "sample": f"oracle (unfused C restatement, OpenMP) on the full {'x'.join(map(str, pa.ext_shape))} grid, "
                       f"first {steps} of {len(pa.tgrid)} time steps, {dt:.1f} s"}
 
 
@@ -156,7 +178,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nt", type=int, default=4000)
+    ap.add_argument("--nt", type=int, default=0, help="time steps (default: the workload's own)")
+    ap.add_argument("--workload", choices=["elastic2d", "acoustic3d"], default="elastic2d")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-shots", type=int, default=512)
@@ -176,13 +199,20 @@ def main():
     import geophyinv as G
     from geophyinv import _lib, gallery
 
-    med, ageom, srcwav, tgrid, kw = gallery.elastic_layers_2d(nt=args.nt)
+    three = args.workload == "acoustic3d"
+    if three:
+        med, ageom, srcwav, tgrid, kw = gallery.acoustic_gradient_3d(nt=args.nt or 3000)
+        attrib = G.FdtdAcoustic()
+    else:
+        med, ageom, srcwav, tgrid, kw = gallery.elastic_layers_2d(nt=args.nt or 4000)
+        attrib = G.FdtdElastic()
     ctx = _lib.Context(0)
     ctx.set_pass_sampling(64)
-    pa = G.SeisForwExpt(G.FdtdElastic(), medium=med, tgrid=tgrid, ageom=ageom, srcwav=srcwav,
-                        context=ctx, **kw)
-    nz, nx = pa.ext_shape
+    pa = G.SeisForwExpt(attrib, medium=med, tgrid=tgrid, ageom=ageom, srcwav=srcwav, context=ctx, **kw)
+    shape = pa.ext_shape
+    cells = int(np.prod(shape))
     nt = len(tgrid)
+    nt_full = shape == (2081, 2081) and pa.order == 2   # the profiled workload
     for _ in range(args.warmup):
         pa.run_shot(0)
     dev_ms, vel_ms, str_ms = [], [], []
@@ -195,9 +225,12 @@ def main():
         str_ms.append(t["stress_avg_ms"])
     wall = time.perf_counter() - t0
     finite = all(np.all(np.isfinite(r)) for r in recs.values())
-    gpt = nz * nx * nt * args.steps
+    gpt = cells * nt * args.steps
     value = gpt / wall / 1e9
-    vel_b, str_b = algorithmic_bytes(nz, nx, pa.npml, 1)
+    if three:
+        vel_b, str_b = algorithmic_bytes_3d(shape, pa.npml)
+    else:
+        vel_b, str_b = algorithmic_bytes(shape[0], shape[1], pa.npml, 1)
     # Each time step is two launches (velocity pass, stress pass).  Event brackets around a
     # sampled pass add the marker latency, so the measured per-step device time is split
     # between the two passes in the ratio of their bracket means: the per-launch figure is
@@ -207,10 +240,24 @@ def main():
     step_s = float(np.mean(dev_ms)) * 1e-3 / nt
     s_avg = step_s * s_br / (s_br + v_br)
     v_avg = step_s * v_br / (s_br + v_br)
-    achieved = str_b / s_avg / 1e9
+    # the dominant kernel: the 2-D elastic stress pass; in 3-D the velocity pass
+    dom_b, dom_t = (vel_b, v_avg) if three else (str_b, s_avg)
+    other_b, other_t, other_br = (str_b, s_avg, s_br) if three else (vel_b, v_avg, v_br)
+    achieved = dom_b / dom_t / 1e9
     step_bytes = vel_b + str_b
+    if three:
+        metric = "Gpt-updates/s (3-D acoustic FDTD, FP32)"
+        workload = ("3-D acoustic FdtdAcoustic, 400^3 at 10 m, 20-cell CPML (440^3), vp 1500->4000 m/s "
+                    "gradient + three +-8% spheres, 12 Hz Ricker on p, 201x201 surface receivers (p), "
+                    "one shot per step")
+        kname, other_name = "vel3d_acoustic<2> (velocity pass)", "stress_pass"
+    else:
+        metric = "Gpt-updates/s (2-D elastic FDTD, FP32)"
+        workload = ("2-D elastic P-SV FdtdElastic, 2001x2001 at 2.5 m, 40-cell CPML (2081x2081), 8 layers + "
+                    "3% smoothed noise, explosive 20 Hz Ricker, 801 receivers (vx, vz), one shot per step")
+        kname, other_name = "str2d_elastic<2> (stress pass)", "velocity_pass"
     line = {
-        "metric": "Gpt-updates/s (2-D elastic FDTD, FP32)",
+        "metric": metric,
         "value": round(value, 3),
         "unit": "Gpt-updates/s",
         "n_gpus": 1,
@@ -222,21 +269,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "2-D elastic P-SV FdtdElastic, 2001x2001 at 2.5 m, 40-cell CPML "
-                               "(2081x2081), 8 layers + 3% smoothed noise, explosive 20 Hz Ricker, "
-                               "801 receivers (vx, vz), one shot per step",
-                   "time_steps": nt, "grid": [nz, nx], "order": pa.order},
+        "config": {"workload": workload, "time_steps": nt, "grid": list(shape), "order": pa.order},
         "device_ms_per_step": round(float(np.mean(dev_ms)), 3),
-        "device_gpt_per_s": round(nz * nx * nt / (float(np.mean(dev_ms)) * 1e-3) / 1e9, 3),
+        "device_gpt_per_s": round(cells * nt / (float(np.mean(dev_ms)) * 1e-3) / 1e9, 3),
         "step_algorithmic_GBps": round(step_bytes * nt / (float(np.mean(dev_ms)) * 1e-3) / 1e9, 1),
-        "roofline": {"bound": "hbm", "kernel": "str2d_elastic<2> (stress pass)",
+        "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": int(str_b), "avg_launch_ms": round(s_avg * 1e3, 5),
-                     "bracket_ms": round(s_br * 1e3, 5),
-                     "velocity_pass": {"bytes_per_launch": int(vel_b), "avg_launch_ms": round(v_avg * 1e3, 5),
-                                       "bracket_ms": round(v_br * 1e3, 5),
-                                       "achieved": round(vel_b / v_avg / 1e9, 1)}},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic("str2d_elastic") if nt_full else None,
+                     "bytes_per_launch": int(dom_b), "avg_launch_ms": round(dom_t * 1e3, 5),
+                     "bracket_ms": round((v_br if three else s_br) * 1e3, 5),
+                     other_name: {"bytes_per_launch": int(other_b), "avg_launch_ms": round(other_t * 1e3, 5),
+                                  "bracket_ms": round(other_br * 1e3, 5),
+                                  "achieved": round(other_b / other_t / 1e9, 1)}},
         "records_finite": bool(finite),
     }
     if not args.no_cpu:

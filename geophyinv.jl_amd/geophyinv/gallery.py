@@ -128,3 +128,45 @@ def serving_stream(nshots: int = 512, seed: int = 3003, rate: float = 200.0, ds:
         out.append(dict(arrival=float(arrivals[k]), medium=medium, ageom=ageom, srcwav=[src], tgrid=tgrid,
                         kw=dict(rfields=["p", "vz"], npml=npml), side=s, nt=nt))
     return out
+
+
+def acoustic_gradient_3d(n: int = 400, ds: float = 10.0, nt: int = 3000, npml: int = 20, freq: float = 12.0,
+                         nrec_side: int = 201, seed: int = 4004):
+    """3-D acoustic: vp = 1500 + 2500 z/L m/s with three spherical anomalies of +-8 % (centres
+    in the middle half of the cube, radii 150-400 m), rho = 2000; 12 Hz Ricker on p five cells
+    below the surface at the centre; nrec_side^2 surface receivers at 20 m spacing recording
+    p; dt at CFL 0.5."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    L = (n - 1) * ds
+    grid = [Grid.range(0.0, L, length=n) for _ in range(3)]
+    z = grid[0].values()[:, None, None]
+    y = grid[1].values()[None, :, None]
+    x = grid[2].values()[None, None, :]
+    vp = 1500.0 + 2500.0 * z / L + 0.0 * y + 0.0 * x
+    for _ in range(3):
+        c = rng.uniform(0.25 * L, 0.75 * L, 3)
+        r = rng.uniform(150.0, 400.0)
+        a = float(rng.choice([-0.08, 0.08]))
+        inside = (z - c[0]) ** 2 + (y - c[1]) ** 2 + (x - c[2]) ** 2 <= r * r
+        vp = np.where(inside, vp * (1.0 + a), vp)
+    vp = np.asfortranarray(vp, dtype=np.float32)
+    rho = np.full((n, n, n), 2000.0, np.float32, order="F")
+    medium = AcousticMedium(grid, vp, rho)
+    dt = 0.5 * ds / float(vp.max())
+    tgrid = Grid(0.0, dt, nt)
+    span = min(L, (nrec_side - 1) * 20.0)
+    ry, rx = np.meshgrid(np.linspace(L / 2 - span / 2, L / 2 + span / 2, nrec_side),
+                         np.linspace(L / 2 - span / 2, L / 2 + span / 2, nrec_side), indexing="ij")
+    ageom = [AGeomss({"z": [5 * ds], "y": [L / 2], "x": [L / 2]},
+                     {"z": np.zeros(ry.size), "y": ry.ravel(), "x": rx.ravel()})]
+    src = Srcs(1, tgrid, ["p"])
+    src.update(["p"], _long_ricker(freq, tgrid))
+    return medium, ageom, [src], tgrid, dict(rfields=["p"], npml=npml)
+
+
+def _long_ricker(freq: float, tgrid: Grid) -> np.ndarray:
+    """Ricker on tgrid, computed on a grid long enough for its delay and then cut."""
+    need = int(math.ceil(3.0 / freq / tgrid.step)) + 2
+    if tgrid.n >= need:
+        return ricker(freq, tgrid)
+    return ricker(freq, Grid(tgrid.first, tgrid.step, need))[:tgrid.n]
