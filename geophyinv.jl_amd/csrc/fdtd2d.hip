@@ -89,6 +89,24 @@ __device__ __forceinline__ float dx_at(const float* a, int off, int ld)
 // no sources (the trailing half step).
 // ---------------------------------------------------------------------------------------
 
+// Samples of a stencil along z (contiguous) or x (stride ld), loaded without evaluating.
+template <int O>
+__device__ __forceinline__ void load_z(float (&v)[O], const float* a, int off)
+{
+#pragma unroll
+    for (int k = 0; k < O; ++k) v[k] = a[off + k];
+}
+
+template <int O>
+__device__ __forceinline__ void load_x(float (&v)[O], const float* a, int off, int ld)
+{
+#pragma unroll
+    for (int k = 0; k < O; ++k) v[k] = a[off + k * ld];
+}
+
+// Each point function first issues every load its updates need (no store precedes them,
+// so they go out together), then evaluates and stores in the reference's order.
+
 template <int O>
 __device__ __forceinline__ void vel_ac_point(const VelParams2D& P, const PassExtras& X, int64_t it, int ninj, int i,
                                              int j)
@@ -96,12 +114,26 @@ __device__ __forceinline__ void vel_ac_point(const VelParams2D& P, const PassExt
     constexpr int H = O - 1;
     const int ld = P.ld;
     const int bk = (i >> 6) + P.gx * j;
-    // vx[inn] += dtinvavxirho * dpdx, dpdx = D_xi(p) * dxI on (nz-2h) x (nx-h)
-    if (i < P.nz - 2 * H && j < P.nx - H) {
-        float d = dx_at<O>(P.s0, (i + H) + ld * j, ld) * P.dxI;
+    const int o = (i + H) + ld * (j + H);
+    const int c = i + ld * j;
+    const bool ux = i < P.nz - 2 * H && j < P.nx - H;   // vx on (nz-2h) x (nx-h)
+    const bool uz = i < P.nz - H && j < P.nx - 2 * H;   // vz on (nz-h) x (nx-2h)
+    float px[O], pz[O], vxo = 0.0f, vzo = 0.0f, cxo = 0.0f, czo = 0.0f;
+    if (ux) {
+        load_x<O>(px, P.s0, (i + H) + ld * j, ld);
+        vxo = P.vx[o];
+        cxo = P.cx[c];
+    }
+    if (uz) {
+        load_z<O>(pz, P.s0, i + ld * (j + H));
+        vzo = P.vz[o];
+        czo = P.cz[c];
+    }
+    // vx[inn] += dtinvavxirho * dpdx, dpdx = D_xi(p) * dxI
+    if (ux) {
+        float d = stencil<O>(px) * P.dxI;
         d = cpml_x(d, P.qx0, i, j, P.npml, P.pml_mask);
-        const int o = (i + H) + ld * (j + H);
-        float nv = P.vx[o] + P.cx[i + ld * j] * d;
+        float nv = vxo + cxo * d;
         // rigid x faces: the only nonzero ghost mirrors the first/last interior column
         // (dirichlet.jl ighostmin/ighostmax with ifd = 1), taken before the source term
         if ((P.rigid_mask & GPI_FACE(2, 0)) && j == 0) P.vx[(i + H)] = -nv;
@@ -109,12 +141,11 @@ __device__ __forceinline__ void vel_ac_point(const VelParams2D& P, const PassExt
         if (ninj) nv = inject_velocity(X, it, INJ_VX, bk, o, nv, P.rho, P.dt);
         P.vx[o] = nv;
     }
-    // vz[inn] += dtinvavzirho * dpdz, dpdz = D_zi(p) * dzI on (nz-h) x (nx-2h)
-    if (i < P.nz - H && j < P.nx - 2 * H) {
-        float d = dz_at<O>(P.s0, i + ld * (j + H)) * P.dzI;
+    // vz[inn] += dtinvavzirho * dpdz, dpdz = D_zi(p) * dzI
+    if (uz) {
+        float d = stencil<O>(pz) * P.dzI;
         d = cpml_z(d, P.qz0, i, j, P.npml, P.pml_mask);
-        const int o = (i + H) + ld * (j + H);
-        float nv = P.vz[o] + P.cz[i + ld * j] * d;
+        float nv = vzo + czo * d;
         if ((P.rigid_mask & GPI_FACE(0, 0)) && i == 0) P.vz[ld * (j + H)] = -nv;
         if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1) P.vz[(P.nz + H - 1) + ld * (j + H)] = -nv;
         if (ninj) nv = inject_velocity(X, it, INJ_VZ, bk, o, nv, P.rho, P.dt);
@@ -129,12 +160,16 @@ __device__ __forceinline__ void str_ac_point(const StrParams2D& P, const PassExt
     if (i >= P.nz || j >= P.nx) return;
     const int ld = P.ld;
     const int o = i + ld * j;
-    float dvx = dx_at<O>(P.vx, o, ld) * P.dxI;       // D_xa(vx)
-    dvx = cpml_x(dvx, P.qx0, i, j, P.npml, P.pml_mask);
-    float dvz = dz_at<O>(P.vz, o) * P.dzI;           // D_za(vz)
-    dvz = cpml_z(dvz, P.qz0, i, j, P.npml, P.pml_mask);
+    float vxs[O], vzs[O];
+    load_x<O>(vxs, P.vx, o, ld);
+    load_z<O>(vzs, P.vz, o);
     const float c = P.c0[o];
-    float p = P.s0[o] + (dvx + dvz) * c;
+    const float po = P.s0[o];
+    float dvx = stencil<O>(vxs) * P.dxI;             // D_xa(vx)
+    dvx = cpml_x(dvx, P.qx0, i, j, P.npml, P.pml_mask);
+    float dvz = stencil<O>(vzs) * P.dzI;             // D_za(vz)
+    dvz = cpml_z(dvz, P.qz0, i, j, P.npml, P.pml_mask);
+    float p = po + (dvx + dvz) * c;
     // p = p + pv * dtK per source field (muladd_tauii!, source.jl:160-163)
     const int bk = (i >> 6) + P.gx * j;
     for (int q = 0; q < ninj; ++q) {
@@ -152,27 +187,42 @@ __device__ __forceinline__ void vel_el_point(const VelParams2D& P, const PassExt
     constexpr int H = O - 1;
     const int ld = P.ld;
     const int bk = (i >> 6) + P.gx * j;
-    // vx[inn] -= dtinvavxirho*(dtauxxdx + dtauxzdz) on (nz-2h) x (nx-h)
-    if (i < P.nz - 2 * H && j < P.nx - H) {
-        float d1 = dx_at<O>(P.s0, (i + H) + ld * j, ld) * P.dxI;   // D_xi(tauxx)
+    const int o = (i + H) + ld * (j + H);
+    const int c = i + ld * j;
+    const bool ux = i < P.nz - 2 * H && j < P.nx - H;   // vx on (nz-2h) x (nx-h)
+    const bool uz = i < P.nz - H && j < P.nx - 2 * H;   // vz on (nz-h) x (nx-2h)
+    float a1[O], a2[O], b1[O], b2[O], vxo = 0.0f, vzo = 0.0f, cxo = 0.0f, czo = 0.0f;
+    if (ux) {
+        load_x<O>(a1, P.s0, (i + H) + ld * j, ld);   // tauxx along x
+        load_z<O>(a2, P.s2, i + ld * j);             // tauxz along z
+        vxo = P.vx[o];
+        cxo = P.cx[c];
+    }
+    if (uz) {
+        load_x<O>(b1, P.s2, i + ld * j, ld);         // tauxz along x
+        load_z<O>(b2, P.s1, i + ld * (j + H));       // tauzz along z
+        vzo = P.vz[o];
+        czo = P.cz[c];
+    }
+    // vx[inn] -= dtinvavxirho*(dtauxxdx + dtauxzdz)
+    if (ux) {
+        float d1 = stencil<O>(a1) * P.dxI;           // D_xi(tauxx)
         d1 = cpml_x(d1, P.qx0, i, j, P.npml, P.pml_mask);
-        float d2 = dz_at<O>(P.s2, i + ld * j) * P.dzI;              // D_za(tauxz)
+        float d2 = stencil<O>(a2) * P.dzI;           // D_za(tauxz)
         d2 = cpml_z(d2, P.qz1, i, j, P.npml, P.pml_mask);
-        const int o = (i + H) + ld * (j + H);
-        float nv = P.vx[o] - P.cx[i + ld * j] * (d1 + d2);
+        float nv = vxo - cxo * (d1 + d2);
         if ((P.rigid_mask & GPI_FACE(2, 0)) && j == 0) P.vx[(i + H)] = -nv;
         if ((P.rigid_mask & GPI_FACE(2, 1)) && j == P.nx - H - 1) P.vx[(i + H) + ld * (P.nx + H - 1)] = -nv;
         if (ninj) nv = inject_velocity(X, it, INJ_VX, bk, o, nv, P.rho, P.dt);
         P.vx[o] = nv;
     }
-    // vz[inn] -= dtinvavzirho*(dtauxzdx + dtauzzdz) on (nz-h) x (nx-2h)
-    if (i < P.nz - H && j < P.nx - 2 * H) {
-        float d1 = dx_at<O>(P.s2, i + ld * j, ld) * P.dxI;          // D_xa(tauxz)
+    // vz[inn] -= dtinvavzirho*(dtauxzdx + dtauzzdz)
+    if (uz) {
+        float d1 = stencil<O>(b1) * P.dxI;           // D_xa(tauxz)
         d1 = cpml_x(d1, P.qx1, i, j, P.npml, P.pml_mask);
-        float d2 = dz_at<O>(P.s1, i + ld * (j + H)) * P.dzI;        // D_zi(tauzz)
+        float d2 = stencil<O>(b2) * P.dzI;           // D_zi(tauzz)
         d2 = cpml_z(d2, P.qz0, i, j, P.npml, P.pml_mask);
-        const int o = (i + H) + ld * (j + H);
-        float nv = P.vz[o] - P.cz[i + ld * j] * (d1 + d2);
+        float nv = vzo - czo * (d1 + d2);
         if ((P.rigid_mask & GPI_FACE(0, 0)) && i == 0) P.vz[ld * (j + H)] = -nv;
         if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1) P.vz[(P.nz + H - 1) + ld * (j + H)] = -nv;
         if (ninj) nv = inject_velocity(X, it, INJ_VZ, bk, o, nv, P.rho, P.dt);
@@ -187,15 +237,32 @@ __device__ __forceinline__ void str_el_point(const StrParams2D& P, const PassExt
     constexpr int H = O - 1;
     const int ld = P.ld;
     const bool fs = (P.free_mask & GPI_FACE(0, 0)) != 0;
-    if (i < P.nz && j < P.nx) {
-        const int o = i + ld * j;
-        float dvx = dx_at<O>(P.vx, o, ld) * P.dxI;   // D_xa(vx)
+    const int o = i + ld * j;
+    const bool un = i < P.nz && j < P.nx;             // tauxx, tauzz
+    const bool us = i < P.nz - H && j < P.nx - H;     // tauxz on (nz-h) x (nx-h)
+    float ax[O], az[O], bz[O], bx[O];
+    float m = 0.0f, l = 0.0f, s0o = 0.0f, s1o = 0.0f, s2o = 0.0f, c2o = 0.0f;
+    if (un) {
+        load_x<O>(ax, P.vx, o, ld);
+        load_z<O>(az, P.vz, o);
+        m = P.c0[o];
+        l = P.c1[o];
+        s0o = P.s0[o];
+        s1o = P.s1[o];
+    }
+    if (us) {
+        load_z<O>(bz, P.vx, i + ld * (j + H));
+        load_x<O>(bx, P.vz, (i + H) + ld * j, ld);
+        s2o = P.s2[o];
+        c2o = P.c2[o];
+    }
+    if (un) {
+        float dvx = stencil<O>(ax) * P.dxI;          // D_xa(vx)
         dvx = cpml_x(dvx, P.qx0, i, j, P.npml, P.pml_mask);
-        float dvz = dz_at<O>(P.vz, o) * P.dzI;       // D_za(vz)
+        float dvz = stencil<O>(az) * P.dzI;          // D_za(vz)
         dvz = cpml_z(dvz, P.qz0, i, j, P.npml, P.pml_mask);
-        const float m = P.c0[o], l = P.c1[o];
-        float txx = (P.s0[o] - (m * dvx)) - (l * dvz);
-        float tzz = (P.s1[o] - (m * dvz)) - (l * dvx);
+        float txx = (s0o - (m * dvx)) - (l * dvz);
+        float tzz = (s1o - (m * dvz)) - (l * dvx);
         // free surface at zmin, before the sources: tauzz[1,:] = -tauzz[2,:]
         if (fs && i == 1) P.s1[ld * j] = -tzz;
         // tauxx += pv*dtM and tauzz += pv*dtM per source field (source.jl:79-97)
@@ -211,15 +278,14 @@ __device__ __forceinline__ void str_el_point(const StrParams2D& P, const PassExt
         P.s0[o] = txx;
         if (!(fs && i == 0)) P.s1[o] = tzz;
     }
-    // tauxz -= dtavmu*(dvxdz + dvzdx) on (nz-h) x (nx-h)
-    if (i < P.nz - H && j < P.nx - H) {
-        const int o = i + ld * j;
-        float a = dz_at<O>(P.vx, i + ld * (j + H)) * P.dzI;      // D_zi(vx)
+    // tauxz -= dtavmu*(dvxdz + dvzdx)
+    if (us) {
+        float a = stencil<O>(bz) * P.dzI;            // D_zi(vx)
         a = cpml_z(a, P.qz1, i, j, P.npml, P.pml_mask);
-        float c = dx_at<O>(P.vz, (i + H) + ld * j, ld) * P.dxI;  // D_xi(vz)
+        float c = stencil<O>(bx) * P.dxI;            // D_xi(vz)
         c = cpml_x(c, P.qx1, i, j, P.npml, P.pml_mask);
-        const float t = P.s2[o] - P.c2[o] * (a + c);
-        P.s2[o] = (fs && i == 0) ? 0.0f : t;                     // tauxz[1,:] = 0
+        const float t = s2o - c2o * (a + c);
+        P.s2[o] = (fs && i == 0) ? 0.0f : t;         // tauxz[1,:] = 0
     }
 }
 

@@ -67,6 +67,13 @@ __device__ __forceinline__ float dstride(const float* a, int off, int stride)
     return stencil<O>(v);
 }
 
+template <int O>
+__device__ __forceinline__ void load_s(float (&v)[O], const float* a, int off, int stride)
+{
+#pragma unroll
+    for (int m = 0; m < O; ++m) v[m] = a[off + m * stride];
+}
+
 struct Idx3 {
     int i, j, k;
 };
@@ -118,14 +125,32 @@ __global__ __launch_bounds__(256) void vel3d_acoustic(VelParams3D P, PassExtras 
         const int bt = bx + P.gx * (by + P.gy * k);
         const int o = (i + H) + s1 * (j + H) + s2 * (k + H);   // @inn of every velocity
         const int c = i + s1 * j + s2 * k;                     // derivative-grid point
-        // vx on (nz-2h, ny-2h, nx-h): dpdx = D_xi(p)
+        const bool uy = i < P.nz - 2 * H && j < P.ny - H && k < P.nx - 2 * H;
+        const bool uz = i < P.nz - H && j < P.ny - 2 * H && k < P.nx - 2 * H;
+        // loads of the three updates first, then evaluation and stores
+        float py[O], pz[O], vxo = 0.0f, cxo = 0.0f, vyo = 0.0f, cyo = 0.0f, vzo = 0.0f, czo = 0.0f;
         if (colx) {
 #pragma unroll
             for (int m = 0; m < O - 1; ++m) w[m] = w[m + 1];
             w[O - 1] = p[cx0 + s2 * (k + O - 1)];
+            vxo = P.vx[o];
+            cxo = P.cx[c];
+        }
+        if (uy) {
+            load_s<O>(py, p, (i + H) + s1 * j + s2 * (k + H), s1);
+            vyo = P.vy[o];
+            cyo = P.cy[c];
+        }
+        if (uz) {
+            load_s<O>(pz, p, i + s1 * (j + H) + s2 * (k + H), 1);
+            vzo = P.vz[o];
+            czo = P.cz[c];
+        }
+        // vx on (nz-2h, ny-2h, nx-h): dpdx = D_xi(p)
+        if (colx) {
             float d = stencil<O>(w) * P.dxI;
             d = cpml3_x(d, P.qx[0], i, j, k, P.npml, P.pml_mask);
-            float nv = P.vx[o] + P.cx[c] * d;
+            float nv = vxo + cxo * d;
             if ((P.rigid_mask & GPI_FACE(2, 0)) && k == 0) P.vx[(i + H) + s1 * (j + H)] = -nv;
             if ((P.rigid_mask & GPI_FACE(2, 1)) && k == P.nx - H - 1)
                 P.vx[(i + H) + s1 * (j + H) + s2 * (P.nx + H - 1)] = -nv;
@@ -133,10 +158,10 @@ __global__ __launch_bounds__(256) void vel3d_acoustic(VelParams3D P, PassExtras 
             P.vx[o] = nv;
         }
         // vy on (nz-2h, ny-h, nx-2h): dpdy = D_yi(p)
-        if (i < P.nz - 2 * H && j < P.ny - H && k < P.nx - 2 * H) {
-            float d = dstride<O>(p, (i + H) + s1 * j + s2 * (k + H), s1) * P.dyI;
+        if (uy) {
+            float d = stencil<O>(py) * P.dyI;
             d = cpml3_y(d, P.qy[0], i, j, k, P.npml, P.pml_mask);
-            float nv = P.vy[o] + P.cy[c] * d;
+            float nv = vyo + cyo * d;
             if ((P.rigid_mask & GPI_FACE(1, 0)) && j == 0) P.vy[(i + H) + s2 * (k + H)] = -nv;
             if ((P.rigid_mask & GPI_FACE(1, 1)) && j == P.ny - H - 1)
                 P.vy[(i + H) + s1 * (P.ny + H - 1) + s2 * (k + H)] = -nv;
@@ -144,10 +169,10 @@ __global__ __launch_bounds__(256) void vel3d_acoustic(VelParams3D P, PassExtras 
             P.vy[o] = nv;
         }
         // vz on (nz-h, ny-2h, nx-2h): dpdz = D_zi(p)
-        if (i < P.nz - H && j < P.ny - 2 * H && k < P.nx - 2 * H) {
-            float d = dstride<O>(p, i + s1 * (j + H) + s2 * (k + H), 1) * P.dzI;
+        if (uz) {
+            float d = stencil<O>(pz) * P.dzI;
             d = cpml3_z(d, P.qz[0], i, j, k, P.npml, P.pml_mask);
-            float nv = P.vz[o] + P.cz[c] * d;
+            float nv = vzo + czo * d;
             if ((P.rigid_mask & GPI_FACE(0, 0)) && i == 0) P.vz[s1 * (j + H) + s2 * (k + H)] = -nv;
             if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1)
                 P.vz[(P.nz + H - 1) + s1 * (j + H) + s2 * (k + H)] = -nv;
@@ -220,15 +245,42 @@ __global__ __launch_bounds__(256) void vel3d_elastic(VelParams3D P, PassExtras X
     const float *txx = P.s[0], *tyy = P.s[1], *tzz = P.s[2], *txy = P.s[3], *txz = P.s[4], *tyz = P.s[5];
     const int o = (i + H) + s1 * (j + H) + s2 * (k + H);
     const int c = i + s1 * j + s2 * k;
+    const bool ux = i < P.nz - 2 * H && j < P.ny - 2 * H && k < P.nx - H;
+    const bool uy = i < P.nz - 2 * H && j < P.ny - H && k < P.nx - 2 * H;
+    const bool uz = i < P.nz - H && j < P.ny - 2 * H && k < P.nx - 2 * H;
+    // loads of the three updates first, then evaluation and stores
+    float a1[O], a2[O], a3[O], b1[O], b2[O], b3[O], e1[O], e2[O], e3[O];
+    float vxo = 0.0f, cxo = 0.0f, vyo = 0.0f, cyo = 0.0f, vzo = 0.0f, czo = 0.0f;
+    if (ux) {
+        load_s<O>(a1, txx, (i + H) + s1 * (j + H) + s2 * k, s2);
+        load_s<O>(a2, txy, c, s1);
+        load_s<O>(a3, txz, c, 1);
+        vxo = P.vx[o];
+        cxo = P.cx[c];
+    }
+    if (uy) {
+        load_s<O>(b1, txy, c, s2);
+        load_s<O>(b2, tyy, (i + H) + s1 * j + s2 * (k + H), s1);
+        load_s<O>(b3, tyz, c, 1);
+        vyo = P.vy[o];
+        cyo = P.cy[c];
+    }
+    if (uz) {
+        load_s<O>(e1, txz, c, s2);
+        load_s<O>(e2, tyz, c, s1);
+        load_s<O>(e3, tzz, i + s1 * (j + H) + s2 * (k + H), 1);
+        vzo = P.vz[o];
+        czo = P.cz[c];
+    }
     // vx -= dtinvavxirho*((dtauxxdx + dtauxydy) + dtauxzdz) on (nz-2h, ny-2h, nx-h)
-    if (i < P.nz - 2 * H && j < P.ny - 2 * H && k < P.nx - H) {
-        float a1 = dstride<O>(txx, (i + H) + s1 * (j + H) + s2 * k, s2) * P.dxI;   // D_xi(tauxx)
-        a1 = cpml3_x(a1, P.qx[0], i, j, k, P.npml, P.pml_mask);
-        float a2 = dstride<O>(txy, c, s1) * P.dyI;                                 // D_ya(tauxy)
-        a2 = cpml3_y(a2, P.qy[0], i, j, k, P.npml, P.pml_mask);
-        float a3 = dstride<O>(txz, c, 1) * P.dzI;                                  // D_za(tauxz)
-        a3 = cpml3_z(a3, P.qz[0], i, j, k, P.npml, P.pml_mask);
-        float nv = P.vx[o] - P.cx[c] * ((a1 + a2) + a3);
+    if (ux) {
+        float d1 = stencil<O>(a1) * P.dxI;   // D_xi(tauxx)
+        d1 = cpml3_x(d1, P.qx[0], i, j, k, P.npml, P.pml_mask);
+        float d2 = stencil<O>(a2) * P.dyI;   // D_ya(tauxy)
+        d2 = cpml3_y(d2, P.qy[0], i, j, k, P.npml, P.pml_mask);
+        float d3 = stencil<O>(a3) * P.dzI;   // D_za(tauxz)
+        d3 = cpml3_z(d3, P.qz[0], i, j, k, P.npml, P.pml_mask);
+        float nv = vxo - cxo * ((d1 + d2) + d3);
         if ((P.rigid_mask & GPI_FACE(2, 0)) && k == 0) P.vx[(i + H) + s1 * (j + H)] = -nv;
         if ((P.rigid_mask & GPI_FACE(2, 1)) && k == P.nx - H - 1)
             P.vx[(i + H) + s1 * (j + H) + s2 * (P.nx + H - 1)] = -nv;
@@ -236,14 +288,14 @@ __global__ __launch_bounds__(256) void vel3d_elastic(VelParams3D P, PassExtras X
         P.vx[o] = nv;
     }
     // vy -= dtinvavyirho*((dtauxydx + dtauyydy) + dtauyzdz) on (nz-2h, ny-h, nx-2h)
-    if (i < P.nz - 2 * H && j < P.ny - H && k < P.nx - 2 * H) {
-        float a1 = dstride<O>(txy, c, s2) * P.dxI;                                 // D_xa(tauxy)
-        a1 = cpml3_x(a1, P.qx[1], i, j, k, P.npml, P.pml_mask);
-        float a2 = dstride<O>(tyy, (i + H) + s1 * j + s2 * (k + H), s1) * P.dyI;   // D_yi(tauyy)
-        a2 = cpml3_y(a2, P.qy[1], i, j, k, P.npml, P.pml_mask);
-        float a3 = dstride<O>(tyz, c, 1) * P.dzI;                                  // D_za(tauyz)
-        a3 = cpml3_z(a3, P.qz[1], i, j, k, P.npml, P.pml_mask);
-        float nv = P.vy[o] - P.cy[c] * ((a1 + a2) + a3);
+    if (uy) {
+        float d1 = stencil<O>(b1) * P.dxI;   // D_xa(tauxy)
+        d1 = cpml3_x(d1, P.qx[1], i, j, k, P.npml, P.pml_mask);
+        float d2 = stencil<O>(b2) * P.dyI;   // D_yi(tauyy)
+        d2 = cpml3_y(d2, P.qy[1], i, j, k, P.npml, P.pml_mask);
+        float d3 = stencil<O>(b3) * P.dzI;   // D_za(tauyz)
+        d3 = cpml3_z(d3, P.qz[1], i, j, k, P.npml, P.pml_mask);
+        float nv = vyo - cyo * ((d1 + d2) + d3);
         if ((P.rigid_mask & GPI_FACE(1, 0)) && j == 0) P.vy[(i + H) + s2 * (k + H)] = -nv;
         if ((P.rigid_mask & GPI_FACE(1, 1)) && j == P.ny - H - 1)
             P.vy[(i + H) + s1 * (P.ny + H - 1) + s2 * (k + H)] = -nv;
@@ -251,14 +303,14 @@ __global__ __launch_bounds__(256) void vel3d_elastic(VelParams3D P, PassExtras X
         P.vy[o] = nv;
     }
     // vz -= dtinvavzirho*((dtauxzdx + dtauyzdy) + dtauzzdz) on (nz-h, ny-2h, nx-2h)
-    if (i < P.nz - H && j < P.ny - 2 * H && k < P.nx - 2 * H) {
-        float a1 = dstride<O>(txz, c, s2) * P.dxI;                                 // D_xa(tauxz)
-        a1 = cpml3_x(a1, P.qx[2], i, j, k, P.npml, P.pml_mask);
-        float a2 = dstride<O>(tyz, c, s1) * P.dyI;                                 // D_ya(tauyz)
-        a2 = cpml3_y(a2, P.qy[2], i, j, k, P.npml, P.pml_mask);
-        float a3 = dstride<O>(tzz, i + s1 * (j + H) + s2 * (k + H), 1) * P.dzI;   // D_zi(tauzz)
-        a3 = cpml3_z(a3, P.qz[2], i, j, k, P.npml, P.pml_mask);
-        float nv = P.vz[o] - P.cz[c] * ((a1 + a2) + a3);
+    if (uz) {
+        float d1 = stencil<O>(e1) * P.dxI;   // D_xa(tauxz)
+        d1 = cpml3_x(d1, P.qx[2], i, j, k, P.npml, P.pml_mask);
+        float d2 = stencil<O>(e2) * P.dyI;   // D_ya(tauyz)
+        d2 = cpml3_y(d2, P.qy[2], i, j, k, P.npml, P.pml_mask);
+        float d3 = stencil<O>(e3) * P.dzI;   // D_zi(tauzz)
+        d3 = cpml3_z(d3, P.qz[2], i, j, k, P.npml, P.pml_mask);
+        float nv = vzo - czo * ((d1 + d2) + d3);
         if ((P.rigid_mask & GPI_FACE(0, 0)) && i == 0) P.vz[s1 * (j + H) + s2 * (k + H)] = -nv;
         if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1)
             P.vz[(P.nz + H - 1) + s1 * (j + H) + s2 * (k + H)] = -nv;
@@ -282,18 +334,53 @@ __global__ __launch_bounds__(256) void str3d_elastic(StrParams3D P, PassExtras X
     const int s1 = (int)P.s1, s2 = (int)P.s2;
     const bool fs = (P.free_mask & GPI_FACE(0, 0)) != 0;
     const int o = i + s1 * j + s2 * k;
+    const bool un = i < P.nz && j < P.ny;
+    const bool uxz = i < P.nz - H && j < P.ny - 2 * H && k < P.nx - H;
+    const bool uxy = i < P.nz - 2 * H && j < P.ny - H && k < P.nx - H;
+    const bool uyz = i < P.nz - H && j < P.ny - H && k < P.nx - 2 * H;
+    // loads of the four updates first, then evaluation and stores
+    float nx_[O], ny_[O], nz_[O], xz1[O], xz2[O], xy1[O], xy2[O], yz1[O], yz2[O];
+    float m = 0.0f, l = 0.0f, s0o = 0.0f, s1o = 0.0f, s2o = 0.0f;
+    float s4o = 0.0f, c3o = 0.0f, s3o = 0.0f, c2o = 0.0f, s5o = 0.0f, c4o = 0.0f;
+    if (un) {
+        load_s<O>(nx_, P.vx, o, s2);
+        load_s<O>(ny_, P.vy, o, s1);
+        load_s<O>(nz_, P.vz, o, 1);
+        m = P.c[0][o];
+        l = P.c[1][o];
+        s0o = P.s[0][o];
+        s1o = P.s[1][o];
+        s2o = P.s[2][o];
+    }
+    if (uxz) {
+        load_s<O>(xz1, P.vx, i + s1 * (j + H) + s2 * (k + H), 1);
+        load_s<O>(xz2, P.vz, (i + H) + s1 * (j + H) + s2 * k, s2);
+        s4o = P.s[4][o];
+        c3o = P.c[3][o];
+    }
+    if (uxy) {
+        load_s<O>(xy1, P.vx, (i + H) + s1 * j + s2 * (k + H), s1);
+        load_s<O>(xy2, P.vy, (i + H) + s1 * (j + H) + s2 * k, s2);
+        s3o = P.s[3][o];
+        c2o = P.c[2][o];
+    }
+    if (uyz) {
+        load_s<O>(yz1, P.vy, i + s1 * (j + H) + s2 * (k + H), 1);
+        load_s<O>(yz2, P.vz, (i + H) + s1 * j + s2 * (k + H), s1);
+        s5o = P.s[5][o];
+        c4o = P.c[4][o];
+    }
     // normal stresses on the stress grid (compute_stressii!, advance_elastic.jl:155-177)
-    if (i < P.nz && j < P.ny) {
-        float dvx = dstride<O>(P.vx, o, s2) * P.dxI;    // D_xa(vx)
+    if (un) {
+        float dvx = stencil<O>(nx_) * P.dxI;    // D_xa(vx)
         dvx = cpml3_x(dvx, P.qx[0], i, j, k, P.npml, P.pml_mask);
-        float dvy = dstride<O>(P.vy, o, s1) * P.dyI;    // D_ya(vy)
+        float dvy = stencil<O>(ny_) * P.dyI;    // D_ya(vy)
         dvy = cpml3_y(dvy, P.qy[0], i, j, k, P.npml, P.pml_mask);
-        float dvz = dstride<O>(P.vz, o, 1) * P.dzI;     // D_za(vz)
+        float dvz = stencil<O>(nz_) * P.dzI;    // D_za(vz)
         dvz = cpml3_z(dvz, P.qz[0], i, j, k, P.npml, P.pml_mask);
-        const float m = P.c[0][o], l = P.c[1][o];
-        float txx = (P.s[0][o] - (m * dvx)) - (l * (dvy + dvz));
-        float tyy = (P.s[1][o] - (m * dvy)) - (l * (dvx + dvz));
-        float tzz = (P.s[2][o] - (m * dvz)) - (l * (dvy + dvx));
+        float txx = (s0o - (m * dvx)) - (l * (dvy + dvz));
+        float tyy = (s1o - (m * dvy)) - (l * (dvx + dvz));
+        float tzz = (s2o - (m * dvz)) - (l * (dvy + dvx));
         if (fs && i == 1) P.s[2][s1 * j + s2 * k] = -tzz;   // tauzz[1,:,:] = -tauzz[2,:,:]
         for (int q = 0; q < X.ninj; ++q) {
             float acc;
@@ -309,29 +396,29 @@ __global__ __launch_bounds__(256) void str3d_elastic(StrParams3D P, PassExtras X
         if (!(fs && i == 0)) P.s[2][o] = tzz;
     }
     // tauxz -= dtavxzimu*(dvxdz + dvzdx) on (nz-h, ny-2h, nx-h)
-    if (i < P.nz - H && j < P.ny - 2 * H && k < P.nx - H) {
-        float a = dstride<O>(P.vx, i + s1 * (j + H) + s2 * (k + H), 1) * P.dzI;   // D_zi(vx)
+    if (uxz) {
+        float a = stencil<O>(xz1) * P.dzI;      // D_zi(vx)
         a = cpml3_z(a, P.qz[1], i, j, k, P.npml, P.pml_mask);
-        float c = dstride<O>(P.vz, (i + H) + s1 * (j + H) + s2 * k, s2) * P.dxI;  // D_xi(vz)
+        float c = stencil<O>(xz2) * P.dxI;      // D_xi(vz)
         c = cpml3_x(c, P.qx[2], i, j, k, P.npml, P.pml_mask);
-        const float v = P.s[4][o] - P.c[3][o] * (a + c);
+        const float v = s4o - c3o * (a + c);
         P.s[4][o] = (fs && i == 0) ? 0.0f : v;
     }
     // tauxy -= dtavxyimu*(dvxdy + dvydx) on (nz-2h, ny-h, nx-h)
-    if (i < P.nz - 2 * H && j < P.ny - H && k < P.nx - H) {
-        float a = dstride<O>(P.vx, (i + H) + s1 * j + s2 * (k + H), s1) * P.dyI;  // D_yi(vx)
+    if (uxy) {
+        float a = stencil<O>(xy1) * P.dyI;      // D_yi(vx)
         a = cpml3_y(a, P.qy[1], i, j, k, P.npml, P.pml_mask);
-        float c = dstride<O>(P.vy, (i + H) + s1 * (j + H) + s2 * k, s2) * P.dxI;  // D_xi(vy)
+        float c = stencil<O>(xy2) * P.dxI;      // D_xi(vy)
         c = cpml3_x(c, P.qx[1], i, j, k, P.npml, P.pml_mask);
-        P.s[3][o] = P.s[3][o] - P.c[2][o] * (a + c);
+        P.s[3][o] = s3o - c2o * (a + c);
     }
     // tauyz -= dtavyzimu*(dvydz + dvzdy) on (nz-h, ny-h, nx-2h)
-    if (i < P.nz - H && j < P.ny - H && k < P.nx - 2 * H) {
-        float a = dstride<O>(P.vy, i + s1 * (j + H) + s2 * (k + H), 1) * P.dzI;   // D_zi(vy)
+    if (uyz) {
+        float a = stencil<O>(yz1) * P.dzI;      // D_zi(vy)
         a = cpml3_z(a, P.qz[2], i, j, k, P.npml, P.pml_mask);
-        float c = dstride<O>(P.vz, (i + H) + s1 * j + s2 * (k + H), s1) * P.dyI;  // D_yi(vz)
+        float c = stencil<O>(yz2) * P.dyI;      // D_yi(vz)
         c = cpml3_y(c, P.qy[2], i, j, k, P.npml, P.pml_mask);
-        const float v = P.s[5][o] - P.c[4][o] * (a + c);
+        const float v = s5o - c4o * (a + c);
         P.s[5][o] = (fs && i == 0) ? 0.0f : v;
     }
 }
