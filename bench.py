@@ -54,11 +54,11 @@ def algorithmic_bytes(nz, nx, npml, physics):
     return vel, stress
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of a kernel from the committed counter summary (FETCH_SIZE +
+def measured_traffic(summary, kernel):
+    """HBM bytes per launch of a kernel from a committed counter summary (FETCH_SIZE +
     WRITE_SIZE runs, calibrated with a copy of known size), or None."""
     import re
-    path = os.path.join(ROOT, "profiles", "fdtd2d_elastic_2081x2081_traffic.md")
+    path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
     m = re.search(r"^traffic_bytes_per_launch %s (\d+)$" % re.escape(kernel), open(path).read(), re.M)
@@ -212,7 +212,8 @@ def main():
     shape = pa.ext_shape
     cells = int(np.prod(shape))
     nt = len(tgrid)
-    nt_full = shape == (2081, 2081) and pa.order == 2   # the profiled workload
+    # the profiled workloads: their measured traffic per launch is in profiles/
+    profiled = pa.order == 2 and shape in ((2081, 2081), (440, 440, 440))
     for _ in range(args.warmup):
         pa.run_shot(0)
     dev_ms, vel_ms, str_ms = [], [], []
@@ -276,7 +277,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic("str2d_elastic") if nt_full else None,
+                     "traffic": (measured_traffic("fdtd3d_acoustic_440_traffic.md", "vel3d_acoustic") if three
+                                 else measured_traffic("fdtd2d_elastic_2081x2081_traffic.md", "str2d_elastic"))
+                                if profiled else None,
                      "bytes_per_launch": int(dom_b), "avg_launch_ms": round(dom_t * 1e3, 5),
                      "bracket_ms": round((v_br if three else s_br) * 1e3, 5),
                      other_name: {"bytes_per_launch": int(other_b), "avg_launch_ms": round(other_t * 1e3, 5),
