@@ -184,10 +184,7 @@ void gpi_destroy(gpi_ctx* c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (Store* s : c->stores) {
-        free_allocs(c, s->allocs, s->bytes);
-        delete s;
-    }
+    for (Store* s : c->stores) free_store(c, s);
     for (Model* m : c->models) free_model(c, m);
     for (auto& e : c->ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->sample_ev) (void)hipEventDestroy(e);
@@ -301,8 +298,7 @@ int gpi_model_free(gpi_ctx* c, uint64_t model)
     for (size_t k = 0; k < c->stores.size();) {   // stores of this model go with it
         Store* s = c->stores[k];
         if (s->model == m) {
-            free_allocs(c, s->allocs, s->bytes);
-            delete s;
+            free_store(c, s);
             c->stores.erase(c->stores.begin() + k);
         } else {
             ++k;

@@ -64,8 +64,12 @@ struct Model {
     State* st0 = nullptr;              // the state of gpi_forward / gpi_forward_save
 };
 
-// Time-reversal store of one :forward_save shot (src/fdtd/boundary.jl): nt strips of the
-// stored stress fields per axis, plus the final snapshots.
+struct Replay;
+
+// Store of one :forward_save shot.  Time reversal (src/fdtd/boundary.jl, every == 0): nt
+// strips of the stored stress fields per axis, plus the final snapshots.  Checkpoints
+// (every > 0): the whole state after every `every`-th step, the velocities at nt+1/2 and
+// the forward sources (Replay) for recomputing the forward wavefield segment by segment.
 struct Store {
     Model* model = nullptr;
     int64_t nt = 0;
@@ -74,6 +78,9 @@ struct Store {
     float* strip[3][3] = {};           // [field][axis]: nt x strip_size
     int64_t strip_size[3][3] = {};
     float* snap[GPI_NFIELDS] = {};     // pitched snapshots: -stress at nt, v at nt+1/2
+    int64_t every = 0;                 // checkpoint interval (0: boundary strips)
+    std::vector<float*> ckpt;          // state after step k*every - 1, k = 1 .. ceil(nt/every)-1
+    Replay* replay = nullptr;
     std::vector<void*> allocs;
     uint64_t bytes = 0;
 };
@@ -105,6 +112,20 @@ void negate_copy(float* dst, const float* src, int64_t n, hipStream_t s);
 // cur/prev/next: p1, vx1, vz1, p2, vx2, vz2 (pitched slabs); next <- cur after imaging
 void image_acoustic_2d(const Model& m, const float* const cur[6], const float* const prev[6], float* const next[6],
                        float* g_invK, float* g_rho, float dtI, hipStream_t s);
+// Imaging of one adjoint step with the exact forward wavefield (acoustic, 2-D and 3-D).
+// a2/c2/n2: the adjoint p and velocities (x, y, z; y unused in 2-D) before the step, after
+// it, and their copy destination; pa, pb: forward p_{n-1}, p_n; va, vb: forward v_n, v_{n+1}.
+// g_rho (and the velocity arguments) may be null.  Accumulators are FP64.
+struct ImageSets {
+    const float* a2[4];
+    const float* c2[4];
+    float* n2[4];
+    const float* pa;
+    const float* pb;
+    const float* va[3];
+    const float* vb[3];
+};
+void image_acoustic_exact(const Model& m, const ImageSets& S, double* g_invK, double* g_rho, float dtI, hipStream_t s);
 
 // CPML strides and vectors of derivative f with the memory array of state st
 inline Pml pml_of(const Model& m, const State& st, int f)

@@ -262,8 +262,7 @@ int alloc_store(gpi_ctx* c, Model& m, int64_t nt, Store** out)
     s->nt = nt;
     s->nfield = stored_fields(m, s->field);
     auto bail = [&](int rc) {
-        free_allocs(c, s->allocs, s->bytes);
-        delete s;
+        free_store(c, s);
         return rc;
     };
     for (int q = 0; q < s->nfield; ++q) {
@@ -286,6 +285,38 @@ int alloc_store(gpi_ctx* c, Model& m, int64_t nt, Store** out)
     for (int f : {GPI_VX, GPI_VY, GPI_VZ}) {
         if (!field_codes(g.ndim, g.physics, f)) continue;
         int rc = dev_alloc(c, &s->allocs, &s->bytes, (void**)&s->snap[f], sizeof(float) * (size_t)m.slab);
+        if (rc) return bail(rc);
+    }
+    *out = s;
+    return GPI_OK;
+}
+
+// Checkpoint store: ceil(nt/every) - 1 states, the velocities at nt+1/2 and a Replay.
+int alloc_checkpoint_store(gpi_ctx* c, Model& m, int64_t nt, int64_t every, Store** out)
+{
+    const gpi_grid& g = m.g;
+    if (g.physics != 0) return fail(c, GPI_EINVAL, "checkpointed gradients are acoustic only");
+    if (every < 1) return fail(c, GPI_EINVAL, "the checkpoint interval must be positive");
+    Store* s = new Store();
+    s->model = &m;
+    s->nt = nt;
+    s->every = every;
+    s->replay = new Replay();
+    s->replay->S = new Scratch(c);
+    auto bail = [&](int rc) {
+        free_store(c, s);
+        return rc;
+    };
+    const int64_t K = (nt + every - 1) / every;
+    const size_t per = state_floats(m);
+    s->ckpt.assign((size_t)std::max<int64_t>(K - 1, 0), nullptr);
+    for (float*& p : s->ckpt) {
+        const int rc = dev_alloc(c, &s->allocs, &s->bytes, (void**)&p, sizeof(float) * per);
+        if (rc) return bail(rc);
+    }
+    for (int f : {GPI_VX, GPI_VY, GPI_VZ}) {
+        if (!field_codes(g.ndim, g.physics, f)) continue;
+        const int rc = dev_alloc(c, &s->allocs, &s->bytes, (void**)&s->snap[f], sizeof(float) * (size_t)m.slab);
         if (rc) return bail(rc);
     }
     *out = s;
@@ -370,8 +401,16 @@ int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
     Scratch S(c);
     PassExtras XV{}, XS{}, X0{};
     Sources src;
-    int rc = plan_sources(c, m, S, shot->nsrc, shot->src, shot->wavelets, nt, false, XV, XS, src);
+    const bool ckpt = save && save->every > 0;
+    // a checkpoint store keeps the source tables for its replays
+    int rc = plan_sources(c, m, ckpt ? *save->replay->S : S, shot->nsrc, shot->src, shot->wavelets, nt, false, XV,
+                          XS, src);
     if (rc) return rc;
+    if (ckpt) {   // no receivers yet in XV, XS
+        save->replay->xv = XV;
+        save->replay->xs = XS;
+        save->replay->src = src;
+    }
     std::vector<DevPoints> rec;
     if ((rc = plan_receivers(c, m, st, S, shot, XV, XS, rec))) return rc;
     for (int q = 0; q < shot->nfinal; ++q) {
@@ -414,14 +453,20 @@ int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
         if (E) HIPTRY(c, hipEventRecord(E[3], c->stream));
         for (int q : src.sfall) inject(m, st, src.pts[q], it, c->stream);
         launches += 2 + (int64_t)src.vfall.size() + (int64_t)src.sfall.size();
-        if (save) boundary_strips(m, st, *save, it, true, c->stream, &launches);
+        if (ckpt) {
+            const int64_t k = (it + 1) / save->every;
+            if ((it + 1) % save->every == 0 && k >= 1 && k <= (int64_t)save->ckpt.size())
+                if ((rc = copy_state(c, m, st, save->ckpt[k - 1], true, c->stream))) return rc;
+        } else if (save) {
+            boundary_strips(m, st, *save, it, true, c->stream, &launches);
+        }
         for (int q = 0; q < shot->nsnap; ++q)
             if (shot->snap_steps[q] == it)
                 HIPTRY(c, hipMemcpyAsync(snap[q], st.field[shot->snap_field], sizeof(float) * (size_t)m.slab,
                                          hipMemcpyDeviceToDevice, c->stream));
         smp.done(it);
     }
-    if (save)   // boundary_save_snap_tau!: -stress at nt (boundary.jl:268-287)
+    if (save && !ckpt)   // boundary_save_snap_tau!: -stress at nt (boundary.jl:268-287)
         for (int q = 0; q < save->nfield; ++q) {
             negate_copy(save->snap[save->field[q]], st.field[save->field[q]], m.slab, c->stream);
             ++launches;
@@ -473,6 +518,19 @@ int check_shot(gpi_ctx* c, uint64_t model, const gpi_shot* shot, Model** out)
     return GPI_OK;
 }
 
+// FP64 pitched slab -> compact host array of logical size n[3]
+int download_pitched_f64(gpi_ctx* c, const Model& m, double* dst, const double* src, const int64_t n[3])
+{
+    std::vector<double> stage((size_t)m.slab);
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    HIPTRY(c, hipMemcpy(stage.data(), src, sizeof(double) * (size_t)m.slab, hipMemcpyDeviceToHost));
+    for (int64_t ix = 0; ix < n[2]; ++ix)
+        for (int64_t iy = 0; iy < n[1]; ++iy)
+            std::memcpy(dst + n[0] * (iy + n[1] * ix), &stage[(size_t)(m.pitch0 * (iy + m.pitch1 * ix))],
+                        sizeof(double) * (size_t)n[0]);
+    return GPI_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -496,8 +554,7 @@ int gpi_forward_save(gpi_ctx* c, uint64_t model, const gpi_shot* shot, uint64_t*
     if ((rc = alloc_store(c, *m, shot->nt, &s))) return rc;
     if ((rc = run_shot(c, *m, shot, s))) {
         (void)hipStreamSynchronize(c->stream);
-        free_allocs(c, s->allocs, s->bytes);
-        delete s;
+        free_store(c, s);
         return rc;
     }
     c->stores.push_back(s);
@@ -513,8 +570,7 @@ int gpi_store_free(gpi_ctx* c, uint64_t store)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     c->stores.erase(std::find(c->stores.begin(), c->stores.end(), s));
-    free_allocs(c, s->allocs, s->bytes);
-    delete s;
+    free_store(c, s);
     return GPI_OK;
 }
 
@@ -607,6 +663,176 @@ int gpi_adjoint(gpi_ctx* c, uint64_t model, uint64_t store, const gpi_adjoint_sh
     int64_t n3[3];
     shape_of(g.ndim, g.order, g.physics, g.n, GPI_P, n3);
     if ((rc = download_pitched(c, m, g_invK, gK, n3)) || (rc = download_pitched(c, m, g_rho, gR, n3))) return rc;
+    return GPI_OK;
+}
+
+int gpi_forward_checkpoint(gpi_ctx* c, uint64_t model, const gpi_shot* shot, int32_t every, uint64_t* store_out)
+{
+    if (!store_out) return fail(c, GPI_EINVAL, "null argument");
+    *store_out = 0;
+    Model* m = nullptr;
+    int rc = check_shot(c, model, shot, &m);
+    if (rc) return rc;
+    Store* s = nullptr;
+    if ((rc = alloc_checkpoint_store(c, *m, shot->nt, every, &s))) return rc;
+    if ((rc = run_shot(c, *m, shot, s))) {
+        (void)hipStreamSynchronize(c->stream);
+        free_store(c, s);
+        return rc;
+    }
+    c->stores.push_back(s);
+    *store_out = (uint64_t)(uintptr_t)s;
+    return GPI_OK;
+}
+
+int gpi_adjoint_checkpoint(gpi_ctx* c, uint64_t model, uint64_t store, const gpi_adjoint_shot* shot, float dtI,
+                           double* g_invK, double* g_rho)
+{
+    if (!c || !shot || !g_invK) return fail(c, GPI_EINVAL, "null argument");
+    Model* mp = find_model(c, model);
+    if (!mp) return fail(c, GPI_ENOTFOUND, "unknown model handle");
+    Store* sp = find_store(c, store);
+    if (!sp || sp->model != mp) return fail(c, GPI_ENOTFOUND, "unknown store handle for this model");
+    if (sp->every < 1 || !sp->replay) return fail(c, GPI_EINVAL, "the store holds no checkpoints");
+    Model& m = *mp;
+    const Store& sv = *sp;
+    const Replay& R = *sv.replay;
+    const gpi_grid& g = m.g;
+    if (g.physics != 0) return fail(c, GPI_EINVAL, "checkpointed gradients are acoustic only");
+    const int64_t nt = shot->nt;
+    if (nt != sv.nt) return fail(c, GPI_EINVAL, "nt differs from the stored forward run");
+    if (shot->nsrc2 < 0 || shot->nsrc2 > 8 || (shot->nsrc2 > 0 && !shot->src2))
+        return fail(c, GPI_EINVAL, "bad number of adjoint source fields");
+    if (!(dtI > 0.0f) || !std::isfinite(dtI)) return fail(c, GPI_EINVAL, "dtI must be positive");
+    HIPTRY(c, hipSetDevice(c->device));
+    const bool rho = g_rho != nullptr;
+    const State& s1 = *m.st0;
+    Scratch S(c);
+    struct Held {
+        gpi_ctx* c;
+        State st;
+        ~Held()
+        {
+            (void)hipStreamSynchronize(c->stream);
+            free_allocs(c, st.allocs, st.bytes);
+        }
+    } h2{c, State{}};
+    int rc = alloc_state(c, m, &h2.st);
+    if (rc) return rc;
+    const State& s2 = h2.st;
+    PassExtras XV2{}, XS2{};
+    Sources src2;
+    if ((rc = plan_sources(c, m, S, shot->nsrc2, shot->src2, shot->wav2, nt, true, XV2, XS2, src2))) return rc;
+    PassExtras XV1 = R.xv, XS1 = R.xs;
+
+    // velocity components present: x, (y,) z
+    std::vector<int> vf;
+    for (int f : {GPI_VX, GPI_VY, GPI_VZ})
+        if (field_codes(g.ndim, g.physics, f)) vf.push_back(f);
+    const int nv = (int)vf.size();
+    const int64_t C = sv.every;
+    const int64_t K = (nt + C - 1) / C;
+    const int64_t L = std::min(C, nt);
+    const size_t slab = (size_t)m.slab;
+    const size_t bytes = sizeof(float) * slab;
+    // segment buffers: p_{n0-1} .. p_{n1-1} and (for rho) v_{n0} .. v_{n1}
+    float* segP = nullptr;
+    float* segV[3] = {};
+    float* vnext[3] = {};
+    if ((rc = S.alloc(&segP, (size_t)(L + 1) * slab))) return rc;
+    if (rho)
+        for (int d = 0; d < nv; ++d)
+            if ((rc = S.alloc(&segV[d], (size_t)(L + 1) * slab)) || (rc = S.alloc(&vnext[d], slab))) return rc;
+    // adjoint fields before the step, two alternating sets (p, vx, vy, vz)
+    float* tp[2][4] = {};
+    for (auto& set : tp) {
+        if ((rc = S.alloc(&set[0], slab))) return rc;
+        if (rho)
+            for (int d = 0; d < nv; ++d)
+                if ((rc = S.alloc(&set[1 + d], slab))) return rc;
+    }
+    double *gK = nullptr, *gR = nullptr;
+    if ((rc = S.alloc(&gK, slab))) return rc;
+    if (rho && (rc = S.alloc(&gR, slab))) return rc;
+    HIPTRY(c, hipMemsetAsync(gK, 0, sizeof(double) * slab, c->stream));
+    if (rho) HIPTRY(c, hipMemsetAsync(gR, 0, sizeof(double) * slab, c->stream));
+    if ((rc = zero_state(c, m, s2, c->stream))) return rc;
+    for (auto& set : tp)
+        for (float* p : set)
+            if (p) HIPTRY(c, hipMemsetAsync(p, 0, bytes, c->stream));
+    if (rho)   // v_nt: the trailing half step of the stored run
+        for (int d = 0; d < nv; ++d)
+            HIPTRY(c, hipMemcpyAsync(vnext[d], sv.snap[vf[d]], bytes, hipMemcpyDeviceToDevice, c->stream));
+
+    Sampler smp(c, nt);
+    int64_t launches = 0, it = 0;
+    HIPTRY(c, hipEventRecord(c->ev[0], c->stream));
+    for (int64_t k = K - 1; k >= 0; --k) {
+        const int64_t n0 = k * C, n1 = std::min(n0 + C, nt);
+        // the forward state before step n0, then the segment recomputed with its sources
+        if (k == 0) rc = zero_state(c, m, s1, c->stream);
+        else rc = copy_state(c, m, s1, sv.ckpt[k - 1], false, c->stream);
+        if (rc) return rc;
+        HIPTRY(c, hipMemcpyAsync(segP, s1.field[GPI_P], bytes, hipMemcpyDeviceToDevice, c->stream));
+        for (int64_t n = n0; n < n1; ++n) {
+            XV1.it = XS1.it = n;
+            if ((rc = velocity_pass(c, m, s1, XV1))) return rc;
+            for (int q : R.src.vfall) inject(m, s1, R.src.pts[q], n, c->stream);
+            if ((rc = stress_pass(c, m, s1, XS1))) return rc;
+            for (int q : R.src.sfall) inject(m, s1, R.src.pts[q], n, c->stream);
+            HIPTRY(c, hipMemcpyAsync(segP + (size_t)(n - n0 + 1) * slab, s1.field[GPI_P], bytes,
+                                     hipMemcpyDeviceToDevice, c->stream));
+            if (rho)
+                for (int d = 0; d < nv; ++d)
+                    HIPTRY(c, hipMemcpyAsync(segV[d] + (size_t)(n - n0) * slab, s1.field[vf[d]], bytes,
+                                             hipMemcpyDeviceToDevice, c->stream));
+            launches += 2 + (int64_t)(R.src.vfall.size() + R.src.sfall.size());
+        }
+        if (rho)
+            for (int d = 0; d < nv; ++d)
+                HIPTRY(c, hipMemcpyAsync(segV[d] + (size_t)(n1 - n0) * slab, vnext[d], bytes,
+                                         hipMemcpyDeviceToDevice, c->stream));
+        // the adjoint steps of this segment, forward step n = nt - 1 - it
+        for (int64_t n = n1 - 1; n >= n0; --n, ++it) {
+            hipEvent_t* E = smp.at(it);
+            XV2.it = XS2.it = it;
+            if (E) HIPTRY(c, hipEventRecord(E[0], c->stream));
+            if ((rc = velocity_pass(c, m, s2, XV2))) return rc;
+            if (E) HIPTRY(c, hipEventRecord(E[1], c->stream));
+            for (int q : src2.vfall) inject(m, s2, src2.pts[q], it, c->stream);
+            if (E) HIPTRY(c, hipEventRecord(E[2], c->stream));
+            if ((rc = stress_pass(c, m, s2, XS2))) return rc;
+            if (E) HIPTRY(c, hipEventRecord(E[3], c->stream));
+            ImageSets I{};
+            float* const* prev = tp[it & 1];
+            float* const* next = tp[(it + 1) & 1];
+            I.a2[0] = prev[0];
+            I.c2[0] = s2.field[GPI_P];
+            I.n2[0] = next[0];
+            I.pa = segP + (size_t)(n - n0) * slab;       // p_{n-1}
+            I.pb = segP + (size_t)(n - n0 + 1) * slab;   // p_n
+            if (rho)
+                for (int d = 0; d < nv; ++d) {
+                    const int a = vf[d] - GPI_VX;   // 0 x, 1 y, 2 z
+                    I.a2[1 + a] = prev[1 + d];
+                    I.c2[1 + a] = s2.field[vf[d]];
+                    I.n2[1 + a] = next[1 + d];
+                    I.va[a] = segV[d] + (size_t)(n - n0) * slab;       // v_n
+                    I.vb[a] = segV[d] + (size_t)(n - n0 + 1) * slab;   // v_{n+1}
+                }
+            image_acoustic_exact(m, I, gK, gR, dtI, c->stream);
+            launches += 3 + (int64_t)src2.vfall.size();
+            smp.done(it);
+        }
+        if (rho)
+            for (int d = 0; d < nv; ++d)
+                HIPTRY(c, hipMemcpyAsync(vnext[d], segV[d], bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    if ((rc = finish_timing(c, smp, nt, launches))) return rc;
+    int64_t n3[3];
+    shape_of(g.ndim, g.order, g.physics, g.n, GPI_P, n3);
+    if ((rc = download_pitched_f64(c, m, g_invK, gK, n3))) return rc;
+    if (rho && (rc = download_pitched_f64(c, m, g_rho, gR, n3))) return rc;
     return GPI_OK;
 }
 

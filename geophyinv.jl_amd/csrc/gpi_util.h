@@ -70,6 +70,20 @@ struct Sources {
     std::vector<DevPoints> pts;
     std::vector<int> vfall, sfall;   // sets injected after the velocity / stress pass
 };
+// Forward sources kept by a checkpoint store: replayed, without receivers, when the
+// adjoint run recomputes a segment of the forward wavefield.
+struct Replay {
+    Scratch* S = nullptr;            // owns the device tables
+    PassExtras xv{}, xs{};
+    Sources src;
+};
+// free a store and everything it owns (strips, checkpoints, replayed sources)
+void free_store(gpi_ctx* c, Store* s);
+// floats of one checkpoint: every wavefield and CPML memory array of a state
+size_t state_floats(const Model& m);
+// copy a whole state into (to_ckpt) or out of a checkpoint buffer, on stream s
+int copy_state(gpi_ctx* c, const Model& m, const State& st, float* ckpt, bool to_ckpt, hipStream_t s);
+
 int plan_sources(gpi_ctx* c, const Model& m, Scratch& S, int nsrc, const gpi_points* src, const float* const* wav,
                  int64_t nt, bool velocity_only, PassExtras& XV, PassExtras& XS, Sources& out);
 int plan_receivers(gpi_ctx* c, const Model& m, const State& st, Scratch& S, const gpi_shot* shot, PassExtras& XV,

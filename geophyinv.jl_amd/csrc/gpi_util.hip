@@ -148,6 +148,51 @@ void free_model(gpi_ctx* c, Model* m)
     delete m;
 }
 
+void free_store(gpi_ctx* c, Store* s)
+{
+    if (!s) return;
+    if (s->replay) {
+        delete s->replay->S;   // frees the replayed source tables (after the stream drained)
+        delete s->replay;
+    }
+    free_allocs(c, s->allocs, s->bytes);
+    delete s;
+}
+
+size_t state_floats(const Model& m)
+{
+    size_t n = 0;
+    for (int f = GPI_P; f <= GPI_TAUYZ; ++f)
+        if (field_codes(m.g.ndim, m.g.physics, f)) n += (size_t)m.slab;
+    for (int f = GPI_DPDX; f < GPI_NFIELDS; ++f) n += psi_floats(m, f);
+    return n;
+}
+
+int copy_state(gpi_ctx* c, const Model& m, const State& st, float* ckpt, bool to_ckpt, hipStream_t s)
+{
+    size_t off = 0;
+    auto one = [&](float* a, size_t n) -> int {
+        if (!a) return fail(c, GPI_EINVAL, "internal error: incomplete wavefield state");
+        float* dst = to_ckpt ? ckpt + off : a;
+        const float* src = to_ckpt ? a : ckpt + off;
+        HIPTRY(c, hipMemcpyAsync(dst, src, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        off += n;
+        return GPI_OK;
+    };
+    for (int f = GPI_P; f <= GPI_TAUYZ; ++f)
+        if (field_codes(m.g.ndim, m.g.physics, f)) {
+            const int rc = one(st.field[f], (size_t)m.slab);
+            if (rc) return rc;
+        }
+    for (int f = GPI_DPDX; f < GPI_NFIELDS; ++f) {
+        const size_t n = psi_floats(m, f);
+        if (!n) continue;
+        const int rc = one(st.psi[f], n);
+        if (rc) return rc;
+    }
+    return GPI_OK;
+}
+
 size_t psi_floats(const Model& m, int f)
 {
     const Pml& P = m.pml[f];

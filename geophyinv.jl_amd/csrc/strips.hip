@@ -96,7 +96,83 @@ __global__ __launch_bounds__(256) void image2d_kernel(ImageParams P)
     for (int k = 0; k < 6; ++k) P.next[k][o] = P.cur[k][o];
 }
 
+// Imaging with the exact forward wavefield, one thread per cell of the pitched slab
+// (z rows to the pitch, y columns to the y pitch, x to nx + h): the copy of the adjoint
+// fields into the next set covers every field's extent, the gradients their own ranges.
+// Each step's term is formed in FP32 with the reference's expressions and added to the
+// FP64 accumulators.
+struct ImageExactParams {
+    ImageSets S;
+    double* gK;
+    double* gR;
+    int nz, ny, nx;   // stress grid (ny = 1 in 2-D)
+    int ld;           // z pitch
+    int py;           // y pitch (1 in 2-D)
+    int ncol;         // x extent of the slab (nx + h)
+    int h;
+    int ndim;
+    float dtI;
+};
+
+__global__ __launch_bounds__(256) void image_exact_kernel(ImageExactParams P)
+{
+    const int nbz = (P.ld + 63) / 64;
+    const int i = (int)(blockIdx.x % nbz) * 64 + threadIdx.x;
+    const int64_t jk = (int64_t)(blockIdx.x / nbz) * 4 + threadIdx.y;
+    const int j = (int)(jk % P.py), k = (int)(jk / P.py);
+    if (i >= P.ld || k >= P.ncol) return;
+    const int64_t sy = P.ld, sx = (int64_t)P.ld * P.py;
+    const int64_t o = i + sy * j + sx * k;
+    const ImageSets& S = P.S;
+    if (i < P.nz && j < P.ny && k < P.nx) {
+        // compute_gmodKI!: p2_prev * (p1_prev - p1) * dtI, p1 = -p_forward
+        const float t = (S.a2[0][o] * (S.pa[o] - S.pb[o])) * P.dtI;
+        P.gK[o] = P.gK[o] + (double)t;
+    }
+    const int h = P.h;
+    if (P.gR && i < P.nz - 2 * h && k < P.nx - 2 * h && (P.ndim == 2 || j < P.ny - 2 * h)) {
+        // compute_gmodrho! buffers at the two samples of each @av_*i, then combine_gmodrho!
+        const int jh = P.ndim == 2 ? 0 : j + h;
+        auto buf = [&](int a, int64_t q) { return (S.a2[1 + a][q] * (S.va[a][q] - S.vb[a][q])) * P.dtI; };
+        const int64_t ax = (i + h) + sy * jh + sx * k;     // @av_xi: x pair
+        const int64_t az = i + sy * jh + sx * (k + h);     // @av_zi: z pair
+        const float avx = (buf(0, ax) + buf(0, ax + sx)) * 0.5f;
+        const float avz = (buf(2, az) + buf(2, az + 1)) * 0.5f;
+        const int64_t g = (i + h) + sy * jh + sx * (k + h);
+        double r = P.gR[g] - (double)avx;
+        if (P.ndim == 3) {
+            const int64_t ay = (i + h) + sy * j + sx * (k + h);   // @av_yi: y pair
+            r = r - (double)((buf(1, ay) + buf(1, ay + sy)) * 0.5f);
+        }
+        P.gR[g] = r - (double)avz;
+    }
+    S.n2[0][o] = S.c2[0][o];
+    if (P.gR)
+        for (int a = 0; a < 3; ++a)
+            if (S.n2[1 + a]) S.n2[1 + a][o] = S.c2[1 + a][o];
+}
+
 }  // namespace
+
+void image_acoustic_exact(const Model& m, const ImageSets& S, double* g_invK, double* g_rho, float dtI, hipStream_t s)
+{
+    ImageExactParams P{};
+    P.S = S;
+    P.gK = g_invK;
+    P.gR = g_rho;
+    P.nz = (int)m.g.n[0];
+    P.ny = m.g.ndim == 3 ? (int)m.g.n[1] : 1;
+    P.nx = (int)m.g.n[2];
+    P.ld = (int)m.pitch0;
+    P.py = m.g.ndim == 3 ? (int)m.pitch1 : 1;
+    P.ncol = (int)(m.g.n[2] + m.h);
+    P.h = m.h;
+    P.ndim = m.g.ndim;
+    P.dtI = dtI;
+    const int64_t cols = (int64_t)P.py * P.ncol;
+    const int64_t nb = (int64_t)((P.ld + 63) / 64) * ((cols + 3) / 4);
+    hipLaunchKernelGGL(image_exact_kernel, dim3((unsigned)nb), dim3(64, 4), 0, s, P);
+}
 
 void strip_copy(const Model& m, float* field, const int64_t fn[3], int axis, int64_t np, float* store, bool save,
                 hipStream_t s)

@@ -120,6 +120,10 @@ SIGNATURES = {
     "gpi_adjoint": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(gpi_adjoint_shot), C.c_float,
                               C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "gpi_store_free": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "gpi_forward_checkpoint": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gpi_shot), C.c_int32,
+                                         C.POINTER(C.c_uint64)]),
+    "gpi_adjoint_checkpoint": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(gpi_adjoint_shot), C.c_float,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "gpi_last_timing": (C.c_int, [C.c_void_p, C.POINTER(gpi_timing)]),
     "gpi_set_pass_sampling": (C.c_int, [C.c_void_p, C.c_int32]),
 }
@@ -142,6 +146,11 @@ def load(path: str = LIB_PATH):
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.flags.c_contiguous)
+    return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
 def fptr(a: np.ndarray):

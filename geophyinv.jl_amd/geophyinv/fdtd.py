@@ -110,7 +110,7 @@ class SeisForwExpt:
                  rfields=("vz",), pml_faces=None, rigid_faces=None, stressfree_faces=(),
                  order: int = 2, npml: int | None = None, compat: int = 1, verbose: bool = False,
                  context: "_lib.Context | None" = None, coefficients: str = "host", tsnaps=None,
-                 snaps_field=None):
+                 snaps_field=None, checkpoint: int = 0, grad_rho: bool = True):
         ndim = medium.ndim
         if order not in (2, 4, 6, 8):
             raise ValueError("order must be 2, 4, 6 or 8")
@@ -145,6 +145,13 @@ class SeisForwExpt:
                 raise ValueError("ageom and srcwav mismatch")
             if not a.inside(medium.grid):
                 raise ValueError("sources or receivers not inside medium")
+        # checkpoint > 0: :forward_save keeps the whole state every `checkpoint` steps and
+        # :adjoint recomputes the forward wavefield exactly from it (gradients in FP64, 2-D
+        # and 3-D acoustic); 0: the reference's boundary strips and time reversal
+        if checkpoint < 0 or (checkpoint and physics != 0):
+            raise ValueError("checkpoint must be >= 0, and checkpointed gradients are acoustic only")
+        self.checkpoint = int(checkpoint)
+        self.grad_rho = bool(grad_rho)
         self.attrib_mod = attrib_mod
         self.physics = physics
         self.ndim = ndim
@@ -203,7 +210,8 @@ class SeisForwExpt:
         check_stability(self.medium, float(self.dt32), self.freqmax, order, verbose)
         self._update_weights()
         self.data = [Recs(a.nr, tgrid, rfields) for a in ageom]
-        self.gradients = {k: np.zeros(self.ext_shape, F32, order="F") for k in self.mparams}
+        gdt = np.float64 if self.checkpoint else F32
+        self.gradients = {k: np.zeros(self.ext_shape, gdt, order="F") for k in self.mparams}
         self._ctx = context
         self._model = None
         self._stores = {}
@@ -456,7 +464,11 @@ class SeisForwExpt:
         if save:
             self._free_store(iss)
             h = C.c_uint64()
-            ctx.check(ctx.lib.gpi_forward_save(ctx.h, self._model, C.byref(shot), C.byref(h)))
+            if self.checkpoint:
+                ctx.check(ctx.lib.gpi_forward_checkpoint(ctx.h, self._model, C.byref(shot), self.checkpoint,
+                                                         C.byref(h)))
+            else:
+                ctx.check(ctx.lib.gpi_forward_save(ctx.h, self._model, C.byref(shot), C.byref(h)))
             self._stores[iss] = h.value
         else:
             ctx.check(ctx.lib.gpi_forward(ctx.h, self._model, C.byref(shot)))
@@ -466,8 +478,9 @@ class SeisForwExpt:
     def run_adjoint(self, iss: int):
         """:adjoint run of supersource ``iss`` (npw = 2, 2-D acoustic): returns its
         gradients (``pap[1].ss[iss].gradients``) w.r.t. invK and rho on the padded grid."""
-        if self.physics != 0 or self.ndim != 2:
-            raise NotImplementedError("the reference gradient exists for 2-D acoustic only (gradient.jl:17-64)")
+        if self.physics != 0 or (self.ndim != 2 and not self.checkpoint):
+            raise NotImplementedError("the reference gradient exists for 2-D acoustic only (gradient.jl:17-64); "
+                                      "3-D acoustic gradients need checkpoint > 0")
         if iss not in self._stores:
             raise RuntimeError("run the :forward_save mode before the :adjoint mode")
         ctx = self._ctx
@@ -479,9 +492,19 @@ class SeisForwExpt:
         sh.nt = len(self.tgrid)
         sh.nsrc1, sh.src1, sh.wav1 = n1, s1, w1
         sh.nsrc2, sh.src2, sh.wav2 = n2, s2, w2
+        dtI = F32(1.0 / self.tgrid.step)   # fc[:dtI] (fdtd.jl:326)
+        if self.checkpoint:
+            gk = np.zeros(self.ext_shape, np.float64, order="F")
+            gr = np.zeros(self.ext_shape, np.float64, order="F") if self.grad_rho else None
+            ctx.check(ctx.lib.gpi_adjoint_checkpoint(ctx.h, self._model, self._stores[iss], C.byref(sh), float(dtI),
+                                                     _lib.dptr(gk), _lib.dptr(gr) if gr is not None else None))
+            self.timing = ctx.timing()
+            out = {"invK": gk}
+            if gr is not None:
+                out["rho"] = gr
+            return out
         gk = np.zeros(self.ext_shape, F32, order="F")
         gr = np.zeros(self.ext_shape, F32, order="F")
-        dtI = F32(1.0 / self.tgrid.step)   # fc[:dtI] (fdtd.jl:326)
         ctx.check(ctx.lib.gpi_adjoint(ctx.h, self._model, self._stores[iss], C.byref(sh), float(dtI),
                                       _lib.fptr(gk), _lib.fptr(gr)))
         self.timing = ctx.timing()
@@ -504,7 +527,8 @@ class SeisForwExpt:
             for iss in range(len(self.ageom)):
                 gi = self.run_adjoint(iss)
                 for k, g in gi.items():
-                    self.gradients[k][...] = self.gradients[k] + g
+                    if k in self.gradients:
+                        self.gradients[k][...] = self.gradients[k] + g
             return self.timing
         for iss in range(len(self.ageom)):
             recs, _ = self.run_shot(iss, save=(mode == "forward_save"))

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define GPI_ABI_VERSION 2
+#define GPI_ABI_VERSION 3
 
 /* error codes */
 #define GPI_OK 0
@@ -234,6 +234,32 @@ typedef struct gpi_adjoint_shot {
 int gpi_adjoint(gpi_ctx* ctx, uint64_t model, uint64_t store, const gpi_adjoint_shot* shot, float dtI,
                 float* g_invK, float* g_rho);
 int gpi_store_free(gpi_ctx* ctx, uint64_t store);
+
+/* ---- checkpointed gradient (acoustic, 2-D and 3-D) ----------------------------------- */
+/* The :forward_save run of gpi_forward_save with checkpoints in place of boundary strips
+ * (no reference counterpart: the reference keeps strips, src/fdtd/boundary.jl:217-306, and
+ * has no 3-D gradient).  The store keeps the whole wavefield and CPML state after every
+ * `every`-th time step, the velocities at nt+1/2 and the forward sources, so that
+ * gpi_adjoint_checkpoint can recompute the forward wavefield exactly, one segment of
+ * `every` steps at a time.  Same records as gpi_forward.  Memory: (ceil(nt/every) - 1)
+ * wavefield states. */
+int gpi_forward_checkpoint(gpi_ctx* ctx, uint64_t model, const gpi_shot* shot, int32_t every,
+                           uint64_t* store_out);
+
+/* The gradient of one supersource from a checkpoint store.  The adjoint wavefield is
+ * gpi_adjoint's second wavefield (src2/wav2; src1/wav1 are not used: the forward sources are
+ * the stored run's).  The forward wavefield is recomputed from the checkpoints, segment by
+ * segment, instead of being re-propagated backwards, so it is exact up to the PML.  After
+ * adjoint step it (forward step n = nt-1-it), with p_n, v_n the forward fields after step n
+ * and v_nt the trailing half step:
+ *   g_invK += p2_prev * (p_{n-1} - p_n) * dtI                    (compute_gmodKI!)
+ *   g_rho[inn] -= @av_xi(vx2_prev (vx_n - vx_{n+1}) dtI) [+ @av_yi(vy ...)] + @av_zi(vz ...)
+ * (gradient.jl:17-64 with the first wavefield's sign convention; in 3-D @av_yi joins the
+ * other two as in diff3D.jl).  Each step's term is formed in FP32 as in the reference and
+ * accumulated in FP64.  g_invK, g_rho: host outputs on the stress grid, FP64; g_rho may be
+ * NULL (the velocities are then not kept: 1 instead of 1 + ndim slabs per segment step). */
+int gpi_adjoint_checkpoint(gpi_ctx* ctx, uint64_t model, uint64_t store, const gpi_adjoint_shot* shot,
+                           float dtI, double* g_invK, double* g_rho);
 
 /* ---- serving: continuous batching of many shots on one GPU ------------------------- */
 /* No reference counterpart (the reference runs supersources one after another per worker,

@@ -1171,6 +1171,27 @@ int oracle_run(oracle_t* S, int64_t nsteps)
     return 0;
 }
 
+/* Step `it` of the loop of oracle_run, on the current state (test hook: the exact-wavefield
+ * gradient composes per-step fields of two oracles), and the trailing half step. */
+int oracle_step(oracle_t* S, int64_t it)
+{
+    if (it < 0 || it >= S->e.nt) return -1;
+    record(S, it, 1);
+    velocity_half_step(S);
+    add_velocity_source(S, it);
+    record(S, it, 0);
+    stress_half_step(S);
+    add_stress_source(S, it);
+    S->it_done = it + 1;
+    return 0;
+}
+
+int oracle_trailing(oracle_t* S)
+{
+    velocity_half_step(S);
+    return 0;
+}
+
 int oracle_get_field(const oracle_t* S, int f, float* out, int64_t n)
 {
     if (f < 0 || f >= O_NFIELDS || !S->f[f].a || numel(&S->f[f]) != n) return -1;

@@ -63,6 +63,8 @@ def load():
         for name in ("oracle_get_field", "oracle_get_memory", "oracle_get_coef", "oracle_get_records"):
             getattr(lib, name).argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_int64]
         lib.oracle_run_save.argtypes = [C.c_void_p]
+        lib.oracle_step.argtypes = [C.c_void_p, C.c_int64]
+        lib.oracle_trailing.argtypes = [C.c_void_p]
         lib.oracle_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         lib.oracle_set_wavelets.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_int64]
         lib.oracle_get_store.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int64]
@@ -186,6 +188,15 @@ class Oracle:
         rc = load().oracle_get_field(self.h, FID[f], out.ctypes.data_as(C.POINTER(C.c_float)), out.size)
         assert rc == 0, f
         return out
+
+    def step(self, it):
+        """Time step ``it`` of the loop (record p, velocity pass, velocity sources, record v,
+        stress pass, stress sources) on the current state."""
+        assert load().oracle_step(self.h, it) == 0
+
+    def trailing(self):
+        """The trailing velocity half step (src/fdtd/propagate.jl:254-255)."""
+        assert load().oracle_trailing(self.h) == 0
 
     def run_save(self):
         """:forward_save: forward run keeping the time-reversal store and snapshots."""

@@ -107,6 +107,10 @@ def experiment3d(medium, *, nt=120, sfields=("p",), rfields=("p", "vx", "vy", "v
 ORACLE_COEFS = ("dtinvavxirho", "dtinvavzirho", "dtK", "rho")
 
 
+def _oracle_coefs(pa):
+    return ORACLE_COEFS + (("dtinvavyirho",) if pa.ndim == 3 else ())
+
+
 def oracle_pair(pa, iss, adj_wavelets=None):
     """Oracles of the two wavefields of supersource ``iss`` of a 2-D acoustic npw = 2
     experiment, on ``pa``'s coefficient arrays; the second sprays at the receivers."""
@@ -115,7 +119,7 @@ def oracle_pair(pa, iss, adj_wavelets=None):
 
     a = pa.ageom[iss]
     s1 = O.from_experiment(pa, iss, freqpeak=float(pa.pml_freqpeak))
-    for c in ORACLE_COEFS:
+    for c in _oracle_coefs(pa):
         s1.set_coef(c, pa.coefs[c])
     vfields = [f for f in pa.rfields if f in ("vx", "vy", "vz")]
     nt = len(pa.tgrid)
@@ -125,8 +129,8 @@ def oracle_pair(pa, iss, adj_wavelets=None):
     s2 = O.Oracle(physics=0, order=pa.order, grid=med.grid, vp=med.vp.m, rho=med.rho.m, npml=pa.npml,
                   pml_faces=pa.pml_faces, rigid_faces=pa.rigid_faces, nt=nt, dt=pa.tgrid.step,
                   src_pos=a.rec_points(), src_fields=vfields, wavelets=[adj_wavelets[f] for f in vfields],
-                  rec_pos=a.rec_points(), rec_fields=vfields, freqpeak=float(pa.pml_freqpeak))
-    for c in ORACLE_COEFS:
+                  rec_pos=a.rec_points(), rec_fields=vfields, freqpeak=float(pa.pml_freqpeak), compat=pa.compat)
+    for c in _oracle_coefs(pa):
         s2.set_coef(c, pa.coefs[c])
     return s1, s2, get_source
 
@@ -159,8 +163,80 @@ def oracle_gradient(pa, dobs):
     return J, g_k, g_r, dcal
 
 
+def oracle_gradient_exact(pa, dobs, grad_rho=True, lag=0):
+    """(J, g_invK, g_rho, dcal) with the exact forward wavefield, on the oracle: the forward
+    oracle stepped to keep p_n and v_n after every step (and v at nt+1/2), the adjoint
+    oracle driven by the time-reversed 2 (dobs - dcal) as in the reference flow, and after
+    adjoint step it (n = nt-1-it)
+        g_invK += p2_prev * (p_{n-1} - p_n) * dtI
+        g_rho[inn] -= @av_xi(vx2_prev (vx_n - vx_{n+1}) dtI) (+ @av_yi) + @av_zi(...)
+    with each term in FP32 and the sums in FP64, stacked over supersources in FP64.  ``lag``
+    pairs adjoint step it with forward step n - lag instead (a planted misalignment for
+    tests that check their own sensitivity)."""
+    nt = len(pa.tgrid)
+    h = pa.order - 1
+    dtI = np.float32(1.0 / pa.tgrid.step)
+    shape = pa.ext_shape
+    g_k = np.zeros(shape, np.float64, order="F")
+    g_r = np.zeros(shape, np.float64, order="F")
+    vnames = ["vx", "vz"] if pa.ndim == 2 else ["vx", "vy", "vz"]
+    J = 0.0
+    dcal = []
+    for iss in range(len(pa.ageom)):
+        s1, _, _ = oracle_pair(pa, iss)
+        P, V = [np.zeros(shape, np.float32, order="F")], {f: [] for f in vnames}
+        for n in range(nt):
+            s1.step(n)
+            P.append(s1.field("p"))
+            if grad_rho:
+                for f in vnames:
+                    V[f].append(s1.field(f))
+        s1.trailing()
+        if grad_rho:
+            for f in vnames:
+                V[f].append(s1.field(f))
+        rec = {f: s1.records(q) for q, f in enumerate(pa.rfields)}
+        dcal.append(rec)
+        adj = {}
+        for f in pa.rfields:
+            d = dobs[iss].d[f].astype(np.float64)
+            J += float(np.sum((d - rec[f].astype(np.float64)) ** 2))
+            if f in ("vx", "vy", "vz"):
+                adj[f] = (2.0 * (d - rec[f].astype(np.float64))).astype(np.float32)[::-1].copy()
+        _, s2, _ = oracle_pair(pa, iss, adj)
+        p2 = np.zeros(shape, np.float32, order="F")
+        v2 = {f: np.zeros_like(V[f][0]) for f in vnames} if grad_rho else {}
+        for it in range(nt):
+            n = nt - 1 - it
+            s2.step(it)
+            q = min(max(n - lag, 0), nt - 1)
+            g_k += ((p2 * (P[q] - P[q + 1])) * dtI).astype(np.float64)
+            if grad_rho:
+                b = {f: (v2[f] * (V[f][n] - V[f][n + 1])) * dtI for f in vnames}
+                if pa.ndim == 2:
+                    nz, nx = shape
+                    I, K = slice(0, nz - 2 * h), slice(0, nx - 2 * h)
+                    bx, bz = b["vx"], b["vz"]
+                    avx = (bx[h:nz - h, 0:nx - 2 * h] + bx[h:nz - h, 1:nx - 2 * h + 1]) * np.float32(0.5)
+                    avz = (bz[0:nz - 2 * h, h:nx - h] + bz[1:nz - 2 * h + 1, h:nx - h]) * np.float32(0.5)
+                    inn = (slice(h, nz - h), slice(h, nx - h))
+                    g_r[inn] = (g_r[inn] - avx.astype(np.float64)) - avz.astype(np.float64)
+                else:
+                    nz, ny, nx = shape
+                    bx, by, bz = b["vx"], b["vy"], b["vz"]
+                    zi, yi, xi = slice(h, nz - h), slice(h, ny - h), slice(h, nx - h)
+                    avx = (bx[zi, yi, 0:nx - 2 * h] + bx[zi, yi, 1:nx - 2 * h + 1]) * np.float32(0.5)
+                    avy = (by[zi, 0:ny - 2 * h, xi] + by[zi, 1:ny - 2 * h + 1, xi]) * np.float32(0.5)
+                    avz = (bz[0:nz - 2 * h, yi, xi] + bz[1:nz - 2 * h + 1, yi, xi]) * np.float32(0.5)
+                    inn = (zi, yi, xi)
+                    g_r[inn] = ((g_r[inn] - avx.astype(np.float64)) - avy.astype(np.float64)) - avz.astype(np.float64)
+                v2 = {f: s2.field(f) for f in vnames}
+            p2 = s2.field("p")
+    return J, g_k, g_r, dcal
+
+
 def fwi_experiment(medium, *, nss=2, nt=300, sfields=("vz",), rfields=("vz",), nr=9, order=2, npml=10,
-                   pml_faces=None, mode="forward_save", ctx=None, seed=5):
+                   pml_faces=None, mode="forward_save", ctx=None, seed=5, **extra):
     """A 2-D acoustic npw = 2 experiment with ``nss`` supersources (one source each) and a
     line of receivers below them."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -181,7 +257,31 @@ def fwi_experiment(medium, *, nss=2, nt=300, sfields=("vz",), rfields=("vz",), n
         srcwav.append(s)
     kw = {} if pml_faces is None else {"pml_faces": pml_faces}
     return G.SeisForwExpt(G.FdtdAcoustic(mode, 2), medium=medium, tgrid=tgrid, ageom=ageom, srcwav=srcwav,
-                          rfields=list(rfields), order=order, npml=npml, context=ctx, **kw)
+                          rfields=list(rfields), order=order, npml=npml, context=ctx, **kw, **extra)
+
+
+def fwi_experiment3d(medium, *, nss=1, nt=160, sfields=("vz",), rfields=("vz",), nr=12, order=2, npml=6,
+                     ctx=None, seed=13, **extra):
+    """A 3-D acoustic npw = 2 experiment: ``nss`` supersources (one source each) in the
+    upper part, a patch of receivers below them."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    gz, gy, gx = medium.grid
+    Lz, Ly, Lx = gz.last, gy.last, gx.last
+    dt = 0.3 * gz.step / (1.7 * float(medium.vp.m.max()))
+    tgrid = Grid(0.0, dt, nt)
+    ageom, srcwav = [], []
+    for k in range(nss):
+        s = {"z": [0.25 * Lz + rng.uniform(-1, 1) * gz.step], "y": [Ly * rng.uniform(0.35, 0.65)],
+             "x": [Lx * rng.uniform(0.35, 0.65)]}
+        r = {"z": np.full(nr, 0.7 * Lz) + rng.uniform(0, 1, nr) * gz.step, "y": Ly * rng.uniform(0.15, 0.85, nr),
+             "x": Lx * rng.uniform(0.15, 0.85, nr)}
+        ag = G.AGeomss(s, r)
+        src = G.Srcs(1, tgrid, list(sfields))
+        src.update(list(sfields), wavelet(nt, dt, 18.0, 0.05 + 0.004 * k) * 1e3)
+        ageom.append(ag)
+        srcwav.append(src)
+    return G.SeisForwExpt(G.FdtdAcoustic("forward_save", 2), medium=medium, tgrid=tgrid, ageom=ageom,
+                          srcwav=srcwav, rfields=list(rfields), order=order, npml=npml, context=ctx, **extra)
 
 
 def oracle_records(medium, pa_like):
@@ -195,7 +295,7 @@ def oracle_records(medium, pa_like):
                      npml=pa_like.npml, pml_faces=pa_like.pml_faces, rigid_faces=pa_like.rigid_faces,
                      nt=len(pa_like.tgrid), dt=pa_like.tgrid.step, src_pos=a.src_points(),
                      src_fields=s.fields, wavelets=[s.d[f] for f in s.fields], rec_pos=a.rec_points(),
-                     rec_fields=pa_like.rfields)
+                     rec_fields=pa_like.rfields, compat=pa_like.compat)
         o.run()
         out.append(G.Recs(a.nr, pa_like.tgrid, pa_like.rfields,
                           {f: o.records(q) for q, f in enumerate(pa_like.rfields)}))

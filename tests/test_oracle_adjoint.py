@@ -104,3 +104,64 @@ def test_gradient_sign_is_pinned():
     sl = (slice(pa.npml, -pa.npml), slice(pa.npml, -pa.npml))
     corr = np.sum(gk[sl].astype(np.float64) * gk2[sl]) / (np.linalg.norm(gk[sl]) * np.linalg.norm(gk2[sl]))
     assert corr < -0.99
+
+
+# ---------------------------------------------------------------------------------------
+# The exact-wavefield gradient (the checkpointed path's restatement, oracle_gradient_exact)
+# ---------------------------------------------------------------------------------------
+
+def test_exact_gradient_equals_reference_flow_away_from_the_strips():
+    """The time-reversed first wavefield of the reference flow equals the forward wavefield
+    up to FP32 rounding except near the boundary strips, so the two gradients agree there
+    to ~1e-7; a one-step misalignment of the forward fields in the exact gradient would be
+    off by O(1) (checked by shifting)."""
+    from helpers import oracle_gradient_exact
+    true = layered_acoustic(nz=40, nx=56, seed=7)
+    med0 = true.copy()
+    med0.vp.m[18:24, 25:35] *= np.float32(0.97)
+    pa = fwi_experiment(med0, nss=2, nt=300)
+    dobs = records(true, pa)
+    J, gk, gr, _ = oracle_gradient(pa, dobs)
+    J2, gk2, gr2, _ = oracle_gradient_exact(pa, dobs)
+    assert J == J2 and J > 0
+    n = pa.npml + 3
+    inner = (slice(n, -n), slice(n, -n))
+    for a, b in ((gk, gk2), (gr, gr2)):
+        e = np.linalg.norm(a[inner] - b[inner]) / np.linalg.norm(b[inner])
+        assert e < 1e-6, e
+    # teeth: forward fields paired with the adjoint step one step off differ grossly
+    _, gk3, _, _ = oracle_gradient_exact(pa, dobs, grad_rho=False, lag=1)
+    assert np.linalg.norm(gk[inner] - gk3[inner]) / np.linalg.norm(gk2[inner]) > 1e-2
+
+
+def _blob3(shape, c, rad):
+    z, y, x = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+    return np.exp(-((z - c[0]) ** 2 + (y - c[1]) ** 2 + (x - c[2]) ** 2) / (2.0 * rad ** 2))
+
+
+def test_exact_gradient_3d_matches_directional_finite_differences():
+    """3-D: the invK gradient of the exact-wavefield flow equals the central finite
+    difference of the misfit along smooth interior perturbations (1e-3)."""
+    from helpers import fwi_experiment3d, medium3d, oracle_gradient_exact
+    true = medium3d(0, n=(26, 22, 24), seed=5)
+    med0 = true.copy()
+    med0.vp.m[10:16, 8:14, 9:15] *= np.float32(0.95)
+    pa = fwi_experiment3d(med0, nss=1, nt=160, npml=7)
+    dobs = records(true, pa)
+    J, gk, _, _ = oracle_gradient_exact(pa, dobs, grad_rho=False)
+    assert J > 0
+    npml = pa.npml
+    inner = tuple(slice(npml, npml + s) for s in med0.vp.m.shape)
+    vp0 = med0.vp.m.astype(np.float64)
+    invK = 1.0 / (vp0 ** 2 * med0.rho.m.astype(np.float64))
+    eps = 1e-3
+    for c, rad in [((13, 11, 12), 2.5), ((9, 12, 10), 2.0)]:
+        b = _blob3(med0.vp.m.shape, c, rad)
+        Js = []
+        for s in (1, -1):
+            m = med0.copy()
+            m.vp.m[:] = (vp0 * (1 + s * eps * b) ** -0.5).astype(np.float32)
+            Js.append(G.lossvalue(None, dobs, records(m, pa)))
+        fd = (Js[0] - Js[1]) / (2 * eps)
+        ad = float(np.sum(gk[inner] * invK * b))
+        assert abs(ad / fd - 1) < 1e-3, (c, ad, fd)
