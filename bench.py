@@ -173,13 +173,125 @@ def serving_section(nshots, opts, G, _lib):
     }
 
 
+def run_gradient3d(args, G, _lib, gallery):
+    """The 3-D acoustic FWI gradient: 400^3 at 10 m (+20-cell CPML = 440^3), vp 1500->4000
+    m/s with three +-8% spheres, observed vz on a 201 x 201 surface patch from the true
+    model, the gradient w.r.t. log(invK) at the starting model (true vp smoothed by a
+    Gaussian of 5 cells), checkpoints every --checkpoint steps, FP64 gradient accumulation.
+    One bench step is one whole gradient: the forward run keeping checkpoints, the host
+    residual and adjoint sources, and the adjoint run that recomputes each segment."""
+    import numpy as np
+
+    nt = args.nt or 3000
+    t0 = time.perf_counter()
+    true, start, ageom, srcwav, tgrid, kw = gallery.acoustic_fwi_3d(n=args.grid_n, nt=nt,
+                                                                    nrec_side=min(201, args.grid_n // 2 + 1))
+    ctx = _lib.Context(0)
+    pt = G.SeisForwExpt(G.FdtdAcoustic(), medium=true, tgrid=tgrid, ageom=ageom, srcwav=srcwav, context=ctx, **kw)
+    pt.update()
+    dobs = pt.data
+    pt.free()
+    pa = G.SeisForwExpt(G.FdtdAcoustic("forward_save", 2), medium=start, tgrid=tgrid, ageom=ageom, srcwav=srcwav,
+                        context=ctx, checkpoint=args.checkpoint, grad_rho=False, **kw)
+    setup_s = time.perf_counter() - t0
+    mp = ["invK"]
+    loss = G.L2DistLoss()
+    m = pa.model_vector(mp)
+    g = np.zeros_like(m)
+    for _ in range(args.warmup):
+        G.gradient(g, m, loss, dobs, pa, mp)
+    wall, adj_ms = [], []
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        J = G.gradient(g, m, loss, dobs, pa, mp)
+        wall.append(time.perf_counter() - t1)
+        adj_ms.append(pa.timing["total_ms"])
+    # device time of the forward run with checkpoints (same model vector)
+    G.lossvalue(m, loss, dobs, pa, mp)
+    fwd_ms = pa.timing["total_ms"]
+    J = G.lossvalue(m, loss, dobs, pa, mp)
+    shape = pa.ext_shape
+    cells = int(np.prod(shape))
+    K = (nt + args.checkpoint - 1) // args.checkpoint
+    slab = 4 * (-(-(shape[0] + 1) // 32) * 32) * (shape[1] + 1) * (shape[2] + 1)
+    mean_wall = float(np.mean(wall))
+    line = {
+        "metric": "Gpt-updates/s (3-D acoustic FWI gradient, checkpointed, FP32 wavefields, FP64 reductions)",
+        "value": round(3 * cells * nt / mean_wall / 1e9, 3),
+        "unit": "Gpt-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(mean_wall * 1e3, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "3-D acoustic FWI gradient w.r.t. log(invK): 400^3 at 10 m, 20-cell CPML (440^3), "
+                               "true vp 1500->4000 m/s gradient + three +-8% spheres, start = true smoothed by a "
+                               "5-cell Gaussian, 12 Hz Ricker on p, 201x201 surface receivers (vz), "
+                               f"checkpoints every {args.checkpoint} steps",
+                   "time_steps": nt, "grid": list(shape), "order": pa.order},
+        "propagations_per_gradient": 3,
+        "gradient_s": round(mean_wall, 2),
+        "forward_device_ms": round(fwd_ms, 1),
+        "adjoint_device_ms": round(float(np.mean(adj_ms)), 1),
+        "misfit": J,
+        "checkpoint_wavefields_gb": round((K - 1) * 4 * slab / 1e9, 1),
+        "segment_gb": round((args.checkpoint + 1) * slab / 1e9, 1),
+        "setup_s": round(setup_s, 1),
+    }
+    if args.fd_cells:
+        line["fd_check"] = fd_check_3d(args, pa, m, g, dobs, loss, mp, start, G)
+    ctx.close()
+    return line
+
+
+def fd_check_3d(args, pa, m, g, dobs, loss, mp, start, G):
+    """Central finite differences of the misfit along Gaussian bumps (radius 3 cells) of
+    log(invK) centred on seeded interior cells, against the gradient's directional
+    derivative; plain forward runs at m +- eps*b."""
+    import numpy as np
+
+    from geophyinv import database
+
+    rng = np.random.Generator(np.random.PCG64(4004))
+    n = start.vp.m.shape
+    lo = min(60, n[0] // 3)
+    centres = np.stack([rng.integers(lo, s - lo, args.fd_cells) for s in n], axis=1)
+    eps = args.fd_eps
+    mode = pa.attrib_mod.mode
+    pa.attrib_mod.mode = "forward"
+    out = []
+    w = 9
+    for c in centres:
+        b = np.zeros(n, np.float64, order="F")
+        sl = tuple(slice(int(ci) - w, int(ci) + w + 1) for ci in c)
+        zz, yy, xx = np.meshgrid(*[np.arange(-w, w + 1)] * 3, indexing="ij")
+        b[sl] = np.exp(-(zz ** 2 + yy ** 2 + xx ** 2) / (2.0 * 3.0 ** 2))
+        bv = b.ravel(order="F")
+        Js = []
+        for sgn in (1, -1):
+            pa.update_model(m + sgn * eps * bv, mp)
+            pa.update()
+            Js.append(database.lossvalue(loss, dobs, pa.data))
+        fd = (Js[0] - Js[1]) / (2 * eps)
+        ad = float(np.dot(g, bv))
+        out.append({"cell": [int(x) for x in c], "fd": fd, "adjoint": ad, "rel_err": abs(ad / fd - 1) if fd else None})
+        print(json.dumps({"fd": out[-1]}), flush=True)
+    pa.update_model(m, mp)
+    pa.attrib_mod.mode = mode
+    errs = [o["rel_err"] for o in out if o["rel_err"] is not None]
+    return {"eps": eps, "bump_radius_cells": 3, "cells": out,
+            "max_rel_err": max(errs) if errs else None, "median_rel_err": float(np.median(errs)) if errs else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt", type=int, default=0, help="time steps (default: the workload's own)")
-    ap.add_argument("--workload", choices=["elastic2d", "acoustic3d"], default="elastic2d")
+    ap.add_argument("--workload", choices=["elastic2d", "acoustic3d", "gradient3d"], default="elastic2d")
+    ap.add_argument("--checkpoint", type=int, default=100, help="gradient3d: checkpoint interval")
+    ap.add_argument("--fd-cells", type=int, default=0, help="gradient3d: finite-difference checks")
+    ap.add_argument("--fd-eps", type=float, default=1e-2)
+    ap.add_argument("--grid-n", type=int, default=400, help="gradient3d: cells per side of the medium")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-shots", type=int, default=512)
@@ -199,6 +311,9 @@ def main():
     import geophyinv as G
     from geophyinv import _lib, gallery
 
+    if args.workload == "gradient3d":
+        print(json.dumps(run_gradient3d(args, G, _lib, gallery)), flush=True)
+        return
     three = args.workload == "acoustic3d"
     if three:
         med, ageom, srcwav, tgrid, kw = gallery.acoustic_gradient_3d(nt=args.nt or 3000)

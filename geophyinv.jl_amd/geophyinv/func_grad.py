@@ -52,7 +52,10 @@ def gradient(g, m, loss, dobs, pa, mparams=None) -> float:
     for S in pa.srcwav2:
         for f in S.fields:
             S.d[f][:, :] = S.d[f][::-1].copy()
-    pa.update_srcwav([pa.srcwav, pa.srcwav2], [-1, 1])
+    # the reference's first wavefield runs backwards with the type -1 sink (velocity sources
+    # only); the checkpointed adjoint recomputes it from the stored run's own sources, so
+    # any source field works there and the first wavelets are left as they are
+    pa.update_srcwav([pa.srcwav, pa.srcwav2], [1 if pa.checkpoint else -1, 1])
     pa.attrib_mod.mode = "adjoint"
     pa.update()
     x = np.split(np.asarray(m, np.float64), len(mparams))

@@ -164,6 +164,24 @@ def acoustic_gradient_3d(n: int = 400, ds: float = 10.0, nt: int = 3000, npml: i
     return medium, ageom, [src], tgrid, dict(rfields=["p"], npml=npml)
 
 
+def acoustic_fwi_3d(n: int = 400, ds: float = 10.0, nt: int = 3000, npml: int = 20, freq: float = 12.0,
+                    nrec_side: int = 201, sigma_cells: float = 5.0, seed: int = 4004):
+    """The 3-D acoustic FWI gradient problem: the true model of ``acoustic_gradient_3d``,
+    a starting model that is the true velocity smoothed by a Gaussian of ``sigma_cells``
+    cells (edges replicated), the same source and surface receiver patch recording vz
+    (the velocity records drive the adjoint sources).
+
+    Returns (true medium, starting medium, ageom, srcwav, tgrid, kw)."""
+    from scipy.ndimage import gaussian_filter
+
+    true, ageom, srcwav, tgrid, kw = acoustic_gradient_3d(n=n, ds=ds, nt=nt, npml=npml, freq=freq,
+                                                          nrec_side=nrec_side, seed=seed)
+    vp0 = gaussian_filter(true.vp.m.astype(np.float64), sigma_cells, mode="nearest")
+    start = AcousticMedium(true.grid, np.asfortranarray(vp0, dtype=np.float32), true.rho.m.copy())
+    kw = dict(kw, rfields=["vz"])
+    return true, start, ageom, srcwav, tgrid, kw
+
+
 def _long_ricker(freq: float, tgrid: Grid) -> np.ndarray:
     """Ricker on tgrid, computed on a grid long enough for its delay and then cut."""
     need = int(math.ceil(3.0 / freq / tgrid.step)) + 2
