@@ -237,15 +237,18 @@ def run_gradient3d(args, G, _lib, gallery):
         "setup_s": round(setup_s, 1),
     }
     if args.fd_cells:
-        line["fd_check"] = fd_check_3d(args, pa, m, g, dobs, loss, mp, start, G)
+        line["fd_check"] = fd_check_3d(args, pa, m, g, J, dobs, loss, mp, start, G)
     ctx.close()
     return line
 
 
-def fd_check_3d(args, pa, m, g, dobs, loss, mp, start, G):
-    """Central finite differences of the misfit along Gaussian bumps (radius 3 cells) of
-    log(invK) centred on seeded interior cells, against the gradient's directional
-    derivative; plain forward runs at m +- eps*b."""
+def fd_check_3d(args, pa, m, g, J, dobs, loss, mp, start, G):
+    """Finite differences of the misfit along Gaussian bumps (radius 3 cells) of log(invK)
+    centred on seeded interior cells, against the gradient's directional derivative.
+    Fourth-order central differences from plain forward runs at m +- eps*b, m +- 2 eps*b;
+    eps is chosen per bump so that the misfit moves by about fd_target of itself (the
+    misfit of FP32 wavefields carries a relative noise of ~1e-7, so weakly sensitive bumps
+    need a larger step), within [1e-3, 5e-2]."""
     import numpy as np
 
     from geophyinv import database
@@ -254,7 +257,6 @@ def fd_check_3d(args, pa, m, g, dobs, loss, mp, start, G):
     n = start.vp.m.shape
     lo = min(60, n[0] // 3)
     centres = np.stack([rng.integers(lo, s - lo, args.fd_cells) for s in n], axis=1)
-    eps = args.fd_eps
     mode = pa.attrib_mod.mode
     pa.attrib_mod.mode = "forward"
     out = []
@@ -265,19 +267,21 @@ def fd_check_3d(args, pa, m, g, dobs, loss, mp, start, G):
         zz, yy, xx = np.meshgrid(*[np.arange(-w, w + 1)] * 3, indexing="ij")
         b[sl] = np.exp(-(zz ** 2 + yy ** 2 + xx ** 2) / (2.0 * 3.0 ** 2))
         bv = b.ravel(order="F")
-        Js = []
-        for sgn in (1, -1):
-            pa.update_model(m + sgn * eps * bv, mp)
-            pa.update()
-            Js.append(database.lossvalue(loss, dobs, pa.data))
-        fd = (Js[0] - Js[1]) / (2 * eps)
         ad = float(np.dot(g, bv))
-        out.append({"cell": [int(x) for x in c], "fd": fd, "adjoint": ad, "rel_err": abs(ad / fd - 1) if fd else None})
+        eps = min(max(args.fd_target * J / abs(ad), 1e-3), 5e-2) if ad else 1e-2
+        Js = {}
+        for k in (1, -1, 2, -2):
+            pa.update_model(m + k * eps * bv, mp)
+            pa.update()
+            Js[k] = database.lossvalue(loss, dobs, pa.data)
+        fd = (8.0 * (Js[1] - Js[-1]) - (Js[2] - Js[-2])) / (12.0 * eps)
+        out.append({"cell": [int(x) for x in c], "eps": eps, "fd": fd, "adjoint": ad,
+                    "rel_err": abs(ad / fd - 1) if fd else None})
         print(json.dumps({"fd": out[-1]}), flush=True)
     pa.update_model(m, mp)
     pa.attrib_mod.mode = mode
     errs = [o["rel_err"] for o in out if o["rel_err"] is not None]
-    return {"eps": eps, "bump_radius_cells": 3, "cells": out,
+    return {"fd_target": args.fd_target, "stencil": "4th-order central", "bump_radius_cells": 3, "cells": out,
             "max_rel_err": max(errs) if errs else None, "median_rel_err": float(np.median(errs)) if errs else None}
 
 
@@ -290,7 +294,7 @@ def main():
     ap.add_argument("--workload", choices=["elastic2d", "acoustic3d", "gradient3d"], default="elastic2d")
     ap.add_argument("--checkpoint", type=int, default=100, help="gradient3d: checkpoint interval")
     ap.add_argument("--fd-cells", type=int, default=0, help="gradient3d: finite-difference checks")
-    ap.add_argument("--fd-eps", type=float, default=1e-2)
+    ap.add_argument("--fd-target", type=float, default=1e-3, help="gradient3d: relative misfit change per FD step")
     ap.add_argument("--grid-n", type=int, default=400, help="gradient3d: cells per side of the medium")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
