@@ -49,6 +49,13 @@ struct StrParams3D {
     float dxI, dyI, dzI;
 };
 
+// Rows of the 3-D acoustic velocity pass are shifted by h: workgroup row bx covers
+// derivative-grid rows 64*bx - h .. 64*bx + 63 - h, so the velocity and pressure columns
+// it reads and writes at row i + h start on a 128-byte boundary (no cache line is shared
+// with the neighbouring workgroup, which may run on another XCD).  The density
+// coefficients it reads at row i are stored h floats past their allocation for the same
+// reason.  Returns the shift (0 for elastic).
+int vel3d_row_shift(const Model& m);
 int velocity_blocks_3d(const Model& m, int* gx, int* gy);
 int stress_blocks_3d(const Model& m, int* gx, int* gy);
 int velocity_pass_3d(const Model& m, const State& st, hipStream_t s, const PassExtras& X);

@@ -112,45 +112,57 @@ __global__ __launch_bounds__(256) void vel3d_acoustic(VelParams3D P, PassExtras 
         return;
     }
     const int bx = b % P.gx, r = b / P.gx, by = r % P.gy, ch = r / P.gy;
-    const int i = bx * 64 + (int)threadIdx.x, j = by * 4 + (int)threadIdx.y;
+    const int i = bx * 64 + (int)threadIdx.x - H, j = by * 4 + (int)threadIdx.y;   // rows shifted by h
     const int k0 = ch * P.kc, k1 = min(k0 + P.kc, nxw);
     const int s1 = (int)P.s1, s2 = (int)P.s2;
     const float* p = P.s[0];
-    const bool colx = i < P.nz - 2 * H && j < P.ny - 2 * H;   // this thread updates vx
+    const bool row = i >= 0;
+    const bool colx = row && i < P.nz - 2 * H && j < P.ny - 2 * H;   // this thread updates vx
     const int cx0 = (i + H) + s1 * (j + H);                     // its p column
     float w[O];
 #pragma unroll
     for (int m = 0; m < O - 1; ++m) w[m + 1] = colx ? p[cx0 + s2 * (k0 + m)] : 0.0f;
+    // operands of one plane; the next plane's are loaded before this plane's stores so
+    // that two planes of loads are in flight per wave
+    const bool ycol = row && i < P.nz - 2 * H && j < P.ny - H;   // vy column (plane k < nx-2h)
+    const bool zcol = row && i < P.nz - H && j < P.ny - 2 * H;   // vz column (plane k < nx-2h)
+    struct Ops {
+        float wn, vxo, cxo, vyo, cyo, vzo, czo;
+        float py[O], pz[O];
+    };
+    auto fetch = [&](int k, Ops& L) {
+        const int o = (i + H) + s1 * (j + H) + s2 * (k + H);
+        const int c = i + s1 * j + s2 * k;
+        if (colx) {
+            L.wn = p[cx0 + s2 * (k + O - 1)];
+            L.vxo = P.vx[o];
+            L.cxo = P.cx[c];
+        }
+        if (ycol && k < P.nx - 2 * H) {
+            load_s<O>(L.py, p, (i + H) + s1 * j + s2 * (k + H), s1);
+            L.vyo = P.vy[o];
+            L.cyo = P.cy[c];
+        }
+        if (zcol && k < P.nx - 2 * H) {
+            load_s<O>(L.pz, p, i + s1 * (j + H) + s2 * (k + H), 1);
+            L.vzo = P.vz[o];
+            L.czo = P.cz[c];
+        }
+    };
+    Ops cur{}, nxt{};
+    if (k0 < k1) fetch(k0, cur);
     for (int k = k0; k < k1; ++k) {
+        if (k + 1 < k1) fetch(k + 1, nxt);
         const int bt = bx + P.gx * (by + P.gy * k);
         const int o = (i + H) + s1 * (j + H) + s2 * (k + H);   // @inn of every velocity
-        const int c = i + s1 * j + s2 * k;                     // derivative-grid point
-        const bool uy = i < P.nz - 2 * H && j < P.ny - H && k < P.nx - 2 * H;
-        const bool uz = i < P.nz - H && j < P.ny - 2 * H && k < P.nx - 2 * H;
-        // loads of the three updates first, then evaluation and stores
-        float py[O], pz[O], vxo = 0.0f, cxo = 0.0f, vyo = 0.0f, cyo = 0.0f, vzo = 0.0f, czo = 0.0f;
+        // vx on (nz-2h, ny-2h, nx-h): dpdx = D_xi(p)
         if (colx) {
 #pragma unroll
             for (int m = 0; m < O - 1; ++m) w[m] = w[m + 1];
-            w[O - 1] = p[cx0 + s2 * (k + O - 1)];
-            vxo = P.vx[o];
-            cxo = P.cx[c];
-        }
-        if (uy) {
-            load_s<O>(py, p, (i + H) + s1 * j + s2 * (k + H), s1);
-            vyo = P.vy[o];
-            cyo = P.cy[c];
-        }
-        if (uz) {
-            load_s<O>(pz, p, i + s1 * (j + H) + s2 * (k + H), 1);
-            vzo = P.vz[o];
-            czo = P.cz[c];
-        }
-        // vx on (nz-2h, ny-2h, nx-h): dpdx = D_xi(p)
-        if (colx) {
+            w[O - 1] = cur.wn;
             float d = stencil<O>(w) * P.dxI;
             d = cpml3_x(d, P.qx[0], i, j, k, P.npml, P.pml_mask);
-            float nv = vxo + cxo * d;
+            float nv = cur.vxo + cur.cxo * d;
             if ((P.rigid_mask & GPI_FACE(2, 0)) && k == 0) P.vx[(i + H) + s1 * (j + H)] = -nv;
             if ((P.rigid_mask & GPI_FACE(2, 1)) && k == P.nx - H - 1)
                 P.vx[(i + H) + s1 * (j + H) + s2 * (P.nx + H - 1)] = -nv;
@@ -158,10 +170,10 @@ __global__ __launch_bounds__(256) void vel3d_acoustic(VelParams3D P, PassExtras 
             P.vx[o] = nv;
         }
         // vy on (nz-2h, ny-h, nx-2h): dpdy = D_yi(p)
-        if (uy) {
-            float d = stencil<O>(py) * P.dyI;
+        if (ycol && k < P.nx - 2 * H) {
+            float d = stencil<O>(cur.py) * P.dyI;
             d = cpml3_y(d, P.qy[0], i, j, k, P.npml, P.pml_mask);
-            float nv = vyo + cyo * d;
+            float nv = cur.vyo + cur.cyo * d;
             if ((P.rigid_mask & GPI_FACE(1, 0)) && j == 0) P.vy[(i + H) + s2 * (k + H)] = -nv;
             if ((P.rigid_mask & GPI_FACE(1, 1)) && j == P.ny - H - 1)
                 P.vy[(i + H) + s1 * (P.ny + H - 1) + s2 * (k + H)] = -nv;
@@ -169,16 +181,17 @@ __global__ __launch_bounds__(256) void vel3d_acoustic(VelParams3D P, PassExtras 
             P.vy[o] = nv;
         }
         // vz on (nz-h, ny-2h, nx-2h): dpdz = D_zi(p)
-        if (uz) {
-            float d = stencil<O>(pz) * P.dzI;
+        if (zcol && k < P.nx - 2 * H) {
+            float d = stencil<O>(cur.pz) * P.dzI;
             d = cpml3_z(d, P.qz[0], i, j, k, P.npml, P.pml_mask);
-            float nv = vzo + czo * d;
+            float nv = cur.vzo + cur.czo * d;
             if ((P.rigid_mask & GPI_FACE(0, 0)) && i == 0) P.vz[s1 * (j + H) + s2 * (k + H)] = -nv;
             if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1)
                 P.vz[(P.nz + H - 1) + s1 * (j + H) + s2 * (k + H)] = -nv;
             if (X.ninj) nv = inject_velocity(X, INJ_VZ, bt, o, nv, P.rho, P.dt);
             P.vz[o] = nv;
         }
+        cur = nxt;
     }
 }
 
@@ -201,26 +214,43 @@ __global__ __launch_bounds__(256) void str3d_acoustic(StrParams3D P, PassExtras 
     float w[O];
 #pragma unroll
     for (int m = 0; m < O - 1; ++m) w[m + 1] = P.vx[col + s2 * (k0 + m)];
+    // operands of one plane; the next plane's are loaded before this plane's store
+    struct Ops {
+        float wn, c, p;
+        float vy[O], vz[O];
+    };
+    auto fetch = [&](int k, Ops& L) {
+        const int o = col + s2 * k;
+        L.wn = P.vx[o + s2 * (O - 1)];
+        load_s<O>(L.vy, P.vy, o, s1);
+        load_s<O>(L.vz, P.vz, o, 1);
+        L.c = P.c[0][o];
+        L.p = P.s[0][o];
+    };
+    Ops cur{}, nxt{};
+    if (k0 < k1) fetch(k0, cur);
     for (int k = k0; k < k1; ++k) {
+        if (k + 1 < k1) fetch(k + 1, nxt);
         const int bt = bx + P.gx * (by + P.gy * k);
         const int o = col + s2 * k;
 #pragma unroll
         for (int m = 0; m < O - 1; ++m) w[m] = w[m + 1];
-        w[O - 1] = P.vx[o + s2 * (O - 1)];
+        w[O - 1] = cur.wn;
         float dvx = stencil<O>(w) * P.dxI;                // D_xa(vx)
         dvx = cpml3_x(dvx, P.qx[0], i, j, k, P.npml, P.pml_mask);
-        float dvy = dstride<O>(P.vy, o, s1) * P.dyI;     // D_ya(vy)
+        float dvy = stencil<O>(cur.vy) * P.dyI;          // D_ya(vy)
         dvy = cpml3_y(dvy, P.qy[0], i, j, k, P.npml, P.pml_mask);
-        float dvz = dstride<O>(P.vz, o, 1) * P.dzI;      // D_za(vz)
+        float dvz = stencil<O>(cur.vz) * P.dzI;          // D_za(vz)
         dvz = cpml3_z(dvz, P.qz[0], i, j, k, P.npml, P.pml_mask);
-        const float c = P.c[0][o];
-        float pn = P.s[0][o] + ((dvx + dvz) + dvy) * c;  // compute_p! 3-D, advance_acou.jl:310-313
+        const float c = cur.c;
+        float pn = cur.p + ((dvx + dvz) + dvy) * c;       // compute_p! 3-D, advance_acou.jl:310-313
         for (int q = 0; q < X.ninj; ++q) {
             float acc;
             int tt;
             if (X.inj[q].kind == INJ_P && injection_sum(X.inj[q], bt, o, X.it, acc, tt)) pn = pn + (acc * c);
         }
         P.s[0][o] = pn;
+        cur = nxt;
     }
 }
 
@@ -423,8 +453,8 @@ __global__ __launch_bounds__(256) void str3d_elastic(StrParams3D P, PassExtras X
     }
 }
 
-// x-planes per workgroup of the acoustic passes: GPI_3D_MARCH planes (default 8; 1 gives
-// one plane per workgroup), at most the axis length
+// x-planes per workgroup of the acoustic passes: GPI_3D_MARCH planes (default 4, measured
+// on 440^3 against 2, 8 and 16; 1 gives one plane per workgroup), at most the axis length
 // XCD-aware workgroup order of the 3-D passes: GPI_XCD_REMAP3 (default 0: measured on
 // 440^3 acoustic and 360^3 elastic, the plain order is 3-5 % faster in 3-D)
 int remap3()
@@ -442,7 +472,7 @@ int march_planes(int n)
     static int kc = -1;
     if (kc < 0) {
         const char* e = getenv("GPI_3D_MARCH");
-        kc = e ? atoi(e) : 8;
+        kc = e ? atoi(e) : 4;
         if (kc < 1) kc = 1;
     }
     return std::min(kc, std::max(n, 1));
@@ -452,7 +482,7 @@ template <int O>
 void launch_vel3(const Model& m, hipStream_t s, VelParams3D P, const PassExtras& X)
 {
     constexpr int H = O - 1;
-    P.gx = (P.nz - H + 63) / 64;
+    P.gx = (P.nz - H + vel3d_row_shift(m) + 63) / 64;
     P.gy = (P.ny - H + 3) / 4;
     P.kc = m.g.physics == 0 ? march_planes(P.nx - H) : 1;
     P.remap = remap3();
@@ -475,10 +505,12 @@ void launch_str3(const Model& m, hipStream_t s, StrParams3D P, const PassExtras&
 
 }  // namespace
 
+int vel3d_row_shift(const Model& m) { return m.g.physics == 0 ? m.h : 0; }
+
 int velocity_blocks_3d(const Model& m, int* gx, int* gy)
 {
     const int H = m.h;
-    *gx = (int)((m.g.n[0] - H + 63) / 64);
+    *gx = (int)((m.g.n[0] - H + vel3d_row_shift(m) + 63) / 64);
     *gy = (int)((m.g.n[1] - H + 3) / 4);
     return *gx * *gy * (int)(m.g.n[2] - H);
 }

@@ -218,7 +218,7 @@ int gpi_model_create(gpi_ctx* c, const gpi_grid* g, const gpi_array* coefs, int3
         if (A.count != n3[0] * n3[1] * n3[2] || !A.data)
             return bail(fail(c, GPI_EINVAL, "coefficient " + std::to_string(A.id) + " has the wrong size"));
         if (!m->coef[A.id]) {
-            rc = dev_alloc(c, &m->allocs, &m->bytes, (void**)&m->coef[A.id], sizeof(float) * m->slab);
+            rc = alloc_coef(c, *m, A.id);
             if (rc) return bail(rc);
         }
         rc = upload_pitched(c, *m, m->coef[A.id], A.data, n3);
@@ -279,7 +279,7 @@ int gpi_model_update(gpi_ctx* c, uint64_t model, const gpi_array* coefs, int32_t
         shape_of(m->g.ndim, m->g.order, m->g.physics, m->g.n, grid, n3);
         if (A.count != n3[0] * n3[1] * n3[2] || !A.data) return fail(c, GPI_EINVAL, "coefficient has the wrong size");
         if (!m->coef[A.id]) {
-            int rc = dev_alloc(c, &m->allocs, &m->bytes, (void**)&m->coef[A.id], sizeof(float) * m->slab);
+            int rc = alloc_coef(c, *m, A.id);
             if (rc) return rc;
         }
         int rc = upload_pitched(c, *m, m->coef[A.id], A.data, n3);

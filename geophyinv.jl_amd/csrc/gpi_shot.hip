@@ -88,7 +88,7 @@ int plan_sources(gpi_ctx* c, const Model& m, Scratch& S, int nsrc, const gpi_poi
                 // muladd_with_density_v*! touches @inn(v) only (source.jl:166-177)
                 if (!(iz >= h && iz < n3[0] - h && ix >= h && ix < n3[2] - h)) continue;
                 if (d3 && !(iy >= h && iy < n3[1] - h)) continue;
-                blk = d3 ? (int)((iz - h) / 64 + vgx * ((iy - h) / 4 + vgy * (ix - h)))
+                blk = d3 ? (int)((iz - h + vel3d_row_shift(m)) / 64 + vgx * ((iy - h) / 4 + vgy * (ix - h)))
                          : (int)((iz - h) / 64 + vgx * (ix - h));
             } else {
                 blk = d3 ? (int)(iz / 64 + sgx * (iy / 4 + sgy * ix)) : (int)(iz / 64 + sgx * ix);

@@ -45,6 +45,10 @@ int dev_alloc(gpi_ctx* c, std::vector<void*>* owner, uint64_t* owner_bytes, void
 void free_allocs(gpi_ctx* c, std::vector<void*>& allocs, uint64_t& bytes);
 void free_model(gpi_ctx* c, Model* m);
 
+// device array of coefficient `id` (one pitched slab; the 3-D acoustic density
+// coefficients start vel3d_row_shift floats into their allocation, fdtd3d.h)
+int alloc_coef(gpi_ctx* c, Model& m, int id);
+
 // wavefields + CPML memory of one propagating wavefield of model m
 int alloc_state(gpi_ctx* c, const Model& m, State* st);
 void free_state(gpi_ctx* c, State* st);

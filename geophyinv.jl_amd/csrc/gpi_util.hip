@@ -1,4 +1,5 @@
 // Host-side helpers of the C ABI (field tables, allocation accounting, pitched copies).
+#include "fdtd3d.h"
 #include "gpi_util.h"
 
 namespace gpi {
@@ -118,6 +119,18 @@ int dev_alloc(gpi_ctx* c, std::vector<void*>* owner, uint64_t* owner_bytes, void
     c->used += bytes;
     owner->push_back(*p);
     *owner_bytes += bytes;
+    return GPI_OK;
+}
+
+int alloc_coef(gpi_ctx* c, Model& m, int id)
+{
+    const bool shifted = m.g.ndim == 3 && (id == GPI_C_DTINVAVXIRHO || id == GPI_C_DTINVAVYIRHO ||
+                                           id == GPI_C_DTINVAVZIRHO);
+    const int64_t off = shifted ? vel3d_row_shift(m) : 0;
+    void* base = nullptr;
+    const int rc = dev_alloc(c, &m.allocs, &m.bytes, &base, sizeof(float) * (size_t)(m.slab + off));
+    if (rc) return rc;
+    m.coef[id] = (float*)base + off;
     return GPI_OK;
 }
 

@@ -107,7 +107,7 @@ int prep_coefficients_2d(gpi_ctx* c, Model& m, const float* vp, const float* vs,
     const int nid = el ? 8 : 5;
     for (int k = 0; k < nid; ++k)
         if (!m.coef[ids[k]]) {
-            int rc = dev_alloc(c, &m.allocs, &m.bytes, (void**)&m.coef[ids[k]], sizeof(float) * (size_t)m.slab);
+            int rc = alloc_coef(c, m, ids[k]);
             if (rc) return rc;
         }
     PrepParams P{};
