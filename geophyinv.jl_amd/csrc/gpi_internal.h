@@ -127,6 +127,11 @@ struct ImageSets {
 };
 void image_acoustic_exact(const Model& m, const ImageSets& S, double* g_invK, double* g_rho, float dtI, hipStream_t s);
 
+// Born secondary sources (born.hip): scattered state w2 from background state w1 after the
+// velocity passes (velocity) or after the stress passes; dc: d(dtinvav{x,y,z}irho), dk: d(dtK)
+void born_sources(const Model& m, const State& w1, const State& w2, const float* const dc[3], const float* dk,
+                  bool velocity, hipStream_t s);
+
 // CPML strides and vectors of derivative f with the memory array of state st
 inline Pml pml_of(const Model& m, const State& st, int f)
 {

@@ -836,4 +836,91 @@ int gpi_adjoint_checkpoint(gpi_ctx* c, uint64_t model, uint64_t store, const gpi
     return GPI_OK;
 }
 
+int gpi_born(gpi_ctx* c, uint64_t model, const gpi_shot* shot, const gpi_array* dcoefs, int32_t ndcoefs)
+{
+    Model* mp = nullptr;
+    int rc = check_shot(c, model, shot, &mp);
+    if (rc) return rc;
+    Model& m = *mp;
+    const gpi_grid& g = m.g;
+    if (g.physics != 0) return fail(c, GPI_EINVAL, "Born modelling is acoustic only");
+    if (shot->nfinal != 0 || shot->nsnap != 0) return fail(c, GPI_EINVAL, "Born runs return records only");
+    if (ndcoefs < 0 || (ndcoefs > 0 && !dcoefs)) return fail(c, GPI_EINVAL, "bad perturbation list");
+    const int64_t nt = shot->nt;
+    const State& w1 = *m.st0;
+    Scratch S(c);
+    struct Held {
+        gpi_ctx* c;
+        State st;
+        ~Held()
+        {
+            (void)hipStreamSynchronize(c->stream);
+            free_allocs(c, st.allocs, st.bytes);
+        }
+    } h2{c, State{}};
+    if ((rc = alloc_state(c, m, &h2.st))) return rc;
+    const State& w2 = h2.st;
+    // coefficient perturbations on the device (absent ones are zero)
+    const int ids[4] = {GPI_C_DTINVAVXIRHO, GPI_C_DTINVAVYIRHO, GPI_C_DTINVAVZIRHO, GPI_C_DTK};
+    float* dev[4] = {};
+    for (int q = 0; q < 4; ++q) {
+        if (g.ndim == 2 && ids[q] == GPI_C_DTINVAVYIRHO) continue;
+        if ((rc = S.alloc(&dev[q], (size_t)m.slab))) return rc;
+        HIPTRY(c, hipMemsetAsync(dev[q], 0, sizeof(float) * (size_t)m.slab, c->stream));
+    }
+    for (int a = 0; a < ndcoefs; ++a) {
+        const gpi_array& A = dcoefs[a];
+        int q = -1;
+        for (int t = 0; t < 4; ++t)
+            if (ids[t] == A.id && dev[t]) q = t;
+        if (q < 0) return fail(c, GPI_EINVAL, "perturbation " + std::to_string(A.id) + " is not a Born coefficient");
+        int64_t n3[3];
+        shape_of(g.ndim, g.order, g.physics, g.n, coef_grid(g.ndim, g.physics, A.id), n3);
+        if (A.count != n3[0] * n3[1] * n3[2] || !A.data)
+            return fail(c, GPI_EINVAL, "perturbation " + std::to_string(A.id) + " has the wrong size");
+        HIPTRY(c, hipStreamSynchronize(c->stream));
+        if ((rc = upload_pitched(c, m, dev[q], A.data, n3))) return rc;
+    }
+    const float* dc[3] = {dev[0], dev[1], dev[2]};
+    PassExtras XV1{}, XS1{}, XV2{}, XS2{};
+    Sources src;
+    if ((rc = plan_sources(c, m, S, shot->nsrc, shot->src, shot->wavelets, nt, false, XV1, XS1, src))) return rc;
+    std::vector<DevPoints> rec;
+    if ((rc = plan_receivers(c, m, w2, S, shot, XV2, XS2, rec))) return rc;
+    if ((rc = zero_state(c, m, w1, c->stream)) || (rc = zero_state(c, m, w2, c->stream))) return rc;
+
+    // per step (propagate.jl:170-226 with activepw = [1, 2], src_flags = [true, false],
+    // rec_flags = [false, true]): velocity passes, velocity sources of the background,
+    // velocity Born sources, stress passes, stress sources, stress Born sources
+    Sampler smp(c, nt);
+    int64_t launches = 0;
+    HIPTRY(c, hipEventRecord(c->ev[0], c->stream));
+    for (int64_t it = 0; it < nt; ++it) {
+        hipEvent_t* E = smp.at(it);
+        XV1.it = XS1.it = XV2.it = XS2.it = it;
+        if (E) HIPTRY(c, hipEventRecord(E[0], c->stream));
+        if ((rc = velocity_pass(c, m, w1, XV1)) || (rc = velocity_pass(c, m, w2, XV2))) return rc;
+        if (E) HIPTRY(c, hipEventRecord(E[1], c->stream));
+        for (int q : src.vfall) inject(m, w1, src.pts[q], it, c->stream);
+        born_sources(m, w1, w2, dc, dev[3], true, c->stream);
+        if (E) HIPTRY(c, hipEventRecord(E[2], c->stream));
+        if ((rc = stress_pass(c, m, w1, XS1)) || (rc = stress_pass(c, m, w2, XS2))) return rc;
+        if (E) HIPTRY(c, hipEventRecord(E[3], c->stream));
+        for (int q : src.sfall) inject(m, w1, src.pts[q], it, c->stream);
+        born_sources(m, w1, w2, dc, dev[3], false, c->stream);
+        launches += 6 + (int64_t)(src.vfall.size() + src.sfall.size());
+        smp.done(it);
+    }
+    if ((rc = finish_timing(c, smp, nt, launches))) return rc;
+    for (int q = 0; q < shot->nrec; ++q) {
+        const DevPoints& D = rec[q];
+        std::vector<float> buf((size_t)nt * std::max(D.npts, 1));
+        HIPTRY(c, hipMemcpy(buf.data(), D.rec, sizeof(float) * buf.size(), hipMemcpyDeviceToHost));
+        float* o = shot->records[q];
+        for (int r = 0; r < D.npts; ++r)
+            for (int64_t it = 0; it < nt; ++it) o[(size_t)r * nt + it] = buf[(size_t)it * D.npts + r];
+    }
+    return GPI_OK;
+}
+
 }  // extern "C"

@@ -5,6 +5,7 @@ The names mirror the reference's Julia API: ``SeisForwExpt``, ``update`` (``upda
 (``SrcWav``/``Records``), ``ricker``, ``FdtdAcoustic``/``FdtdElastic``.
 """
 from . import database as _database
+from .born import Born, LinearMap
 from . import func_grad as _func_grad
 from .database import AGeomss, L2DistLoss, Records, Recs, SrcWav, Srcs
 from .fdtd import FdtdAcoustic, FdtdElastic, SeisForwExpt, update
@@ -34,7 +35,7 @@ def gradient(*args, **kw):
 
 
 __all__ = [
-    "AGeom", "AGeomss", "AcousticMedium", "ElasticMedium", "FIELDS", "FIELD_ID", "FdtdAcoustic",
+    "AGeom", "AGeomss", "AcousticMedium", "Born", "ElasticMedium", "LinearMap", "FIELDS", "FIELD_ID", "FdtdAcoustic",
     "FdtdElastic", "Grid", "L2DistLoss", "Medium", "Records", "Recs", "SeisForwExpt", "SeisInvExpt", "SrcWav", "Srcs",
     "field_shape", "findfreq", "get_mgrid", "gradient", "lossvalue", "padarray", "ricker", "update",
 ]

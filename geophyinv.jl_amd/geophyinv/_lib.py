@@ -120,6 +120,7 @@ SIGNATURES = {
     "gpi_adjoint": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(gpi_adjoint_shot), C.c_float,
                               C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "gpi_store_free": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "gpi_born": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gpi_shot), C.POINTER(gpi_array), C.c_int32]),
     "gpi_forward_checkpoint": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gpi_shot), C.c_int32,
                                          C.POINTER(C.c_uint64)]),
     "gpi_adjoint_checkpoint": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(gpi_adjoint_shot), C.c_float,

@@ -33,6 +33,7 @@ FACES = {"zmin": 0, "zmax": 1, "ymin": 2, "ymax": 3, "xmin": 4, "xmax": 5}
 
 class _Attrib:
     physics = -1
+    born = False
 
     def __init__(self, mode: str = "forward", npw: int | None = None):
         if mode not in ("forward", "forward_save", "adjoint"):
@@ -40,8 +41,22 @@ class _Attrib:
         self.mode = mode
         self.npw = npw if npw is not None else (1 if mode == "forward" else 2)
 
+    def __class_getitem__(cls, kind):
+        """``FdtdAcoustic[Born]()`` is ``FdtdAcoustic{Born}()`` (physics_types.jl:38-40):
+        :forward mode with two wavefields."""
+        from .born import Born
+        if kind is not Born:
+            raise TypeError("only Born is a modelling kind")
+
+        def make(mode: str = "forward"):
+            a = cls(mode, 2)
+            a.born = True
+            return a
+        return make
+
     def __repr__(self):
-        return f"{type(self).__name__}(mode={self.mode!r}, npw={self.npw})"
+        kind = "[Born]" if self.born else ""
+        return f"{type(self).__name__}{kind}(mode={self.mode!r}, npw={self.npw})"
 
 
 class FdtdAcoustic(_Attrib):
@@ -152,6 +167,11 @@ class SeisForwExpt:
             raise ValueError("checkpoint must be >= 0, and checkpointed gradients are acoustic only")
         self.checkpoint = int(checkpoint)
         self.grad_rho = bool(grad_rho)
+        self.born = bool(attrib_mod.born)
+        if self.born and (physics != 0 or attrib_mod.npw != 2):
+            raise ValueError("Born modelling is acoustic, with two wavefields")
+        self.dmod = None          # Born: perturbation of mod (invK, rho) on the padded grid
+        self.born_dcoefs = None   # Born: linearised coefficient perturbations
         self.attrib_mod = attrib_mod
         self.physics = physics
         self.ndim = ndim
@@ -522,6 +542,12 @@ class SeisForwExpt:
             d.fill(0.0)
         for g in self.gradients.values():
             g.fill(0.0)
+        if self.born and mode == "forward":   # records of the scattered wavefield
+            from .born import run_born
+            for iss in range(len(self.ageom)):
+                for f, r in run_born(self, iss).items():
+                    self.data[iss].d[f][:, :] = r
+            return self.timing
         if mode == "adjoint":
             self._ensure_model()
             for iss in range(len(self.ageom)):
@@ -589,6 +615,51 @@ class SeisForwExpt:
             return
         self._upload_coefs()
 
+    def set_perturbation(self, dmod: dict):
+        """Born: the perturbation of ``mod`` (``invK``, ``rho``; FP64 on the padded grid,
+        zero outside the unpadded region)."""
+        from .born import dcoefs
+        self.dmod = dmod
+        self.born_dcoefs = dcoefs(self, dmod)
+
+    def update_perturbation(self, medium_pert: Medium):
+        """``update!(pa, medium, medium_pert)`` after the background update (medium.jl:106-116)."""
+        from .born import perturbation
+        self.set_perturbation(perturbation(self, medium_pert))
+
+    def migrate(self, iss: int, adj: dict) -> dict:
+        """The adjoint run of supersource ``iss`` with adjoint sources ``adj`` (field -> nt x
+        nr, already time reversed) at the receivers, after a forward run that keeps its
+        store: the imaging of the background wavefield with the adjoint one (FP64 arrays
+        ``invK``, ``rho`` on the padded grid)."""
+        if self.npw != 2:
+            raise ValueError("migration needs two wavefields")
+        if not self.checkpoint and any(f not in ("vx", "vy", "vz") for f in self.srcwav[iss].fields):
+            raise ValueError("the reference's time reversal needs velocity sources; use checkpoint > 0")
+        saved_w = self.wavelets[iss]
+        self.run_shot(iss, save=True)
+        if not self.checkpoint:   # the reference flow re-propagates with the type -1 sink
+            s = self.srcwav[iss]
+            nt = len(self.tgrid)
+            self.wavelets[iss] = {}
+            for f in s.fields:
+                full = np.zeros((nt, s.n), F32, order="F")
+                w = get_source(s.d[f], f, -1)
+                full[:min(len(s.grid), nt)] = w[:min(len(s.grid), nt)]
+                self.wavelets[iss][f] = full
+        saved_a = self.wavelets2[iss] if iss < len(self.wavelets2) else None
+        self.wavelets2[iss] = {f: np.asfortranarray(a, dtype=F32) for f, a in adj.items()}
+        grad_rho = self.grad_rho
+        self.grad_rho = True
+        try:
+            g = self.run_adjoint(iss)
+        finally:
+            self.grad_rho = grad_rho
+            self.wavelets[iss] = saved_w
+            self.wavelets2[iss] = saved_a
+            self._free_store(iss)
+        return {k: np.asarray(v, np.float64) for k, v in g.items()}
+
     def update_srcwav(self, srcwav, src_types=None):
         """``update!(pa, srcwav, src_types)`` (src/fdtd/source.jl:192-246).
 
@@ -650,7 +721,10 @@ def update(obj, *args, **kw):
     if isinstance(a, np.ndarray):
         return obj.update_model(a, *args[1:])
     if isinstance(a, Medium):
-        return obj.update_medium(a)
+        obj.update_medium(a)
+        if len(args) > 1 and isinstance(args[1], Medium):   # update!(pa, medium, medium_pert)
+            obj.update_perturbation(args[1])
+        return None
     if isinstance(a, list) and a and hasattr(a[0], "ns") and hasattr(a[0], "s"):
         return obj.update_ageom(a)
     return obj.update_srcwav(a, *args[1:], **kw)

@@ -261,6 +261,21 @@ int gpi_forward_checkpoint(gpi_ctx* ctx, uint64_t model, const gpi_shot* shot, i
 int gpi_adjoint_checkpoint(gpi_ctx* ctx, uint64_t model, uint64_t store, const gpi_adjoint_shot* shot,
                            float dtI, double* g_invK, double* g_rho);
 
+/* ---- Born modelling (acoustic, 2-D and 3-D) ------------------------------------------ */
+/* One supersource of FdtdAcoustic{Born} in :forward mode (src/fdtd/born.jl,
+ * src/fdtd/propagate.jl:53-60: both wavefields propagate, the sources drive the background
+ * one, the receivers sample the scattered one).  Per step, after both velocity passes
+ *   v2[inn] += d(dtinvav{x,y,z}irho) * D_{x,y,z}i(p1)
+ * and after both stress passes
+ *   p2 += (D_xa(vx1) + D_za(vz1) [+ D_ya(vy1)]) * d(dtK)
+ * with plain stencil derivatives.  These secondary sources are the linearisation of
+ * compute_v!/compute_p! (advance_acou.jl) in their coefficients; the reference's own Born
+ * calls do not match those kernels' arguments, so there is no reference result to follow.
+ * dcoefs: the coefficient perturbations GPI_C_DTK and GPI_C_DTINVAV{X,(Y,)Z}IRHO on their
+ * grids (host, FP32), zero wherever a CPML memory variable acts.  The shot's nfinal and
+ * nsnap must be 0. */
+int gpi_born(gpi_ctx* ctx, uint64_t model, const gpi_shot* shot, const gpi_array* dcoefs, int32_t ndcoefs);
+
 /* ---- serving: continuous batching of many shots on one GPU ------------------------- */
 /* No reference counterpart (the reference runs supersources one after another per worker,
  * src/fdtd/propagate.jl:100-106).  A server owns a device and a scheduler thread.  Shots

@@ -1186,6 +1186,28 @@ int oracle_step(oracle_t* S, int64_t it)
     return 0;
 }
 
+/* The two halves of oracle_step (test hook: Born modelling adds secondary sources between
+ * them): record p, velocity half step, velocity sources; record v, stress half step,
+ * stress sources. */
+int oracle_step_velocity(oracle_t* S, int64_t it)
+{
+    if (it < 0 || it >= S->e.nt) return -1;
+    record(S, it, 1);
+    velocity_half_step(S);
+    add_velocity_source(S, it);
+    return 0;
+}
+
+int oracle_step_stress(oracle_t* S, int64_t it)
+{
+    if (it < 0 || it >= S->e.nt) return -1;
+    record(S, it, 0);
+    stress_half_step(S);
+    add_stress_source(S, it);
+    S->it_done = it + 1;
+    return 0;
+}
+
 int oracle_trailing(oracle_t* S)
 {
     velocity_half_step(S);

@@ -62,6 +62,8 @@ int oracle_run(oracle_t* S, int64_t nsteps);
 int oracle_run_save(oracle_t* S);
 int oracle_step(oracle_t* S, int64_t it);
 int oracle_trailing(oracle_t* S);
+int oracle_step_velocity(oracle_t* S, int64_t it);
+int oracle_step_stress(oracle_t* S, int64_t it);
 int oracle_adjoint(oracle_t* S1, oracle_t* S2, float* g_invK, float* g_rho);
 int oracle_set_wavelets(oracle_t* S, int q, const float* w, int64_t n);
 int oracle_get_store(const oracle_t* S, int q, int axis, float* out, int64_t n);

@@ -65,6 +65,8 @@ def load():
         lib.oracle_run_save.argtypes = [C.c_void_p]
         lib.oracle_step.argtypes = [C.c_void_p, C.c_int64]
         lib.oracle_trailing.argtypes = [C.c_void_p]
+        lib.oracle_step_velocity.argtypes = [C.c_void_p, C.c_int64]
+        lib.oracle_step_stress.argtypes = [C.c_void_p, C.c_int64]
         lib.oracle_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         lib.oracle_set_wavelets.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_int64]
         lib.oracle_get_store.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int64]
@@ -193,6 +195,14 @@ class Oracle:
         """Time step ``it`` of the loop (record p, velocity pass, velocity sources, record v,
         stress pass, stress sources) on the current state."""
         assert load().oracle_step(self.h, it) == 0
+
+    def step_velocity(self, it):
+        """First half of step ``it``: record p, velocity pass, velocity sources."""
+        assert load().oracle_step_velocity(self.h, it) == 0
+
+    def step_stress(self, it):
+        """Second half of step ``it``: record v, stress pass, stress sources."""
+        assert load().oracle_step_stress(self.h, it) == 0
 
     def trailing(self):
         """The trailing velocity half step (src/fdtd/propagate.jl:254-255)."""

@@ -235,6 +235,52 @@ def oracle_gradient_exact(pa, dobs, grad_rho=True, lag=0):
     return J, g_k, g_r, dcal
 
 
+def oracle_born(pa):
+    """Born records of every supersource on the oracle: the background oracle (sources)
+    and the scattered oracle (no sources, the receivers) stepped half by half; after both
+    velocity halves the scattered velocities receive d(dtinvav*irho) * D(p1), after both
+    stress halves the scattered pressure receives (D(vx1) + D(vz1) [+ D(vy1)]) * d(dtK),
+    in FP32 with the derivatives of the oracle (no CPML memory)."""
+    import oracle as O
+
+    h = pa.order - 1
+    dc = pa.born_dcoefs
+    nt = len(pa.tgrid)
+    out = []
+    d3 = pa.ndim == 3
+    vnames = ["vx", "vy", "vz"] if d3 else ["vx", "vz"]
+    for iss, a in enumerate(pa.ageom):
+        s1 = O.from_experiment(pa, iss, freqpeak=float(pa.pml_freqpeak))
+        med = pa.medium
+        zero = [np.zeros((nt, a.ns), np.float32) for _ in pa.srcwav[iss].fields]
+        s2 = O.Oracle(physics=0, order=pa.order, grid=med.grid, vp=med.vp.m, rho=med.rho.m, npml=pa.npml,
+                      pml_faces=pa.pml_faces, rigid_faces=pa.rigid_faces, nt=nt, dt=pa.tgrid.step,
+                      src_pos=a.src_points(), src_fields=pa.srcwav[iss].fields, wavelets=zero,
+                      rec_pos=a.rec_points(), rec_fields=pa.rfields, freqpeak=float(pa.pml_freqpeak),
+                      compat=pa.compat)
+        for s in (s1, s2):
+            for c in _oracle_coefs(pa):
+                s.set_coef(c, pa.coefs[c])
+        for it in range(nt):
+            s1.step_velocity(it)
+            s2.step_velocity(it)
+            for v in vnames:
+                ax = v[1]
+                d = s1.derivative(f"dpd{ax}")
+                f = s2.field(v)
+                inn = tuple(slice(h, h + n) for n in d.shape)
+                f[inn] = f[inn] + dc[f"dtinvav{ax}irho"] * d
+                s2.set_field(v, f)
+            s1.step_stress(it)
+            s2.step_stress(it)
+            dv = s1.derivative("dvxdx") + s1.derivative("dvzdz")
+            if d3:
+                dv = dv + s1.derivative("dvydy")
+            s2.set_field("p", s2.field("p") + dv * dc["dtK"])
+        out.append({f: s2.records(q) for q, f in enumerate(pa.rfields)})
+    return out
+
+
 def fwi_experiment(medium, *, nss=2, nt=300, sfields=("vz",), rfields=("vz",), nr=9, order=2, npml=10,
                    pml_faces=None, mode="forward_save", ctx=None, seed=5, **extra):
     """A 2-D acoustic npw = 2 experiment with ``nss`` supersources (one source each) and a
