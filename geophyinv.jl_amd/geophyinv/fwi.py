@@ -132,6 +132,29 @@ class SeisInvExpt:
         g[...] = np.concatenate(out)
         return J
 
+    def update_source(self, maxiter: int = 100) -> dict:
+        """``update!(paw, pac)`` (src/fwi/fwi.jl:37-64): model the data at the current model,
+        then per supersource estimate the filter w that best maps the original observed data
+        onto the modelled data (LSMR on the convolution operator with g = dobs0, all fields'
+        traces sharing one w) and replace the observed data by dobs0 * w.  Returns the
+        misfits before and after."""
+        from .conv import estimate_filter
+
+        m = self.get_modelvector()
+        loss_initial = self.lossvalue(m)
+        for iss, (d0, d, dcal) in enumerate(zip(self.dobs0, self.dobs, self.paf.data)):
+            fields = list(dcal.fields)
+            g = np.concatenate([np.asarray(d0.d[f], np.float64) for f in fields], axis=1)
+            dc = np.concatenate([np.asarray(dcal.d[f], np.float64) for f in fields], axis=1)
+            _, conv = estimate_filter(g, dc, maxiter=maxiter)
+            off = 0
+            for f in fields:
+                n = d.d[f].shape[1]
+                d.d[f][:, :] = conv[:, off:off + n].astype(d.d[f].dtype)
+                off += n
+        loss_final = self.lossvalue(m)
+        return {"loss_initial": loss_initial, "loss_final": loss_final}
+
     def update(self, maxiter: int = 20, callback=None):
         """``update!(paw)`` (src/fwi/fwi.jl:66-75): L-BFGS on the misfit from the current
         model.  The reference drives Optim.jl's LBFGS; this uses SciPy's L-BFGS-B without
