@@ -139,3 +139,22 @@ def test_linear_map_dot_product(gpu_ctx, checkpoint):
         print(f"  checkpoint {checkpoint} {'invK' if part == 0 else 'rho'}: <Fx,y> {lhs:.6e} <x,F^T y> {rhs:.6e} "
               f"rel {abs(lhs - rhs) / abs(lhs):.2e}", flush=True)
         assert abs(lhs - rhs) <= tol * abs(lhs)
+
+
+def test_linear_map_additivity_and_normal_operator(gpu_ctx):
+    """test/fwi/born_map.jl: F (x1 + x2) = F x1 + F x2 (to FP32 rounding here) and
+    <x, F^T F x> > 0."""
+    pa = born2d(gpu_ctx, nss=1, rfields=("vz",), checkpoint=40)
+    F = G.LinearMap(pa)
+    rng = np.random.Generator(np.random.PCG64(5))
+    n = F.shape[1] // 2
+    scale = np.concatenate([np.full(n, 1e-12), np.full(n, 1.0)])   # invK ~ 1e-10, rho ~ 1e3
+    x1 = rng.standard_normal(F.shape[1]) * scale
+    x2 = rng.standard_normal(F.shape[1]) * scale
+    d12 = F @ (x1 + x2)
+    d1, d2 = F @ x1, F @ x2
+    e = np.linalg.norm(d12 - (d1 + d2)) / np.linalg.norm(d12)
+    print(f"  additivity rel {e:.2e}", flush=True)
+    assert e < 1e-5
+    c = float(np.dot(x1, F.T @ (F @ x1)))
+    assert c > 0.0
