@@ -132,8 +132,11 @@ def lossvalue(loss, dobs, dcal) -> float:
     L = _loss(loss)
     if isinstance(dobs, list):
         return float(sum(lossvalue(L, a, b) for a, b in zip(dobs, dcal)))
-    return float(sum(np.sum(L.value(dobs.d[f].astype(np.float64), dcal.d[f].astype(np.float64)))
-                     for f in dobs.fields))
+    total = 0.0
+    for f in dobs.fields:
+        r = np.subtract(dobs.d[f], dcal.d[f], dtype=np.float64).ravel(order="K")
+        total += float(np.dot(r, r))   # sum of L2DistLoss values, FP64
+    return total
 
 
 def gradient(buffer, loss, dobs, dcal):
@@ -149,6 +152,8 @@ def gradient(buffer, loss, dobs, dcal):
             gradient(g, L, a, b)
         return buffer
     for f in dobs.fields:
-        buffer.d[f][:, :] = L.deriv(dobs.d[f].astype(np.float64), dcal.d[f].astype(np.float64)).astype(np.float32)
+        # 2 (dobs - dcal): the FP32 difference rounds once, as the FP64 one cast to FP32 does
+        np.subtract(dobs.d[f], dcal.d[f], out=buffer.d[f], dtype=np.float32)
+        buffer.d[f] *= np.float32(2.0)
     return buffer
 

@@ -22,8 +22,8 @@ from .database import Recs, Srcs
 from .fields import (COEF_ID, FIELD_ID, coef_grid, derivative_fields, field_shape, get_mgrid,
                      has_field, wavefields_of)
 from .grid import Grid
-from .media import (MPARAMS, AcousticMedium, ElasticMedium, Medium, MediumParameter, dependent, independent,
-                    padarray)
+from .media import (MPARAMS, AcousticMedium, ElasticMedium, Medium, MediumParameter, dependent, dtk_of,
+                    independent, padarray)
 from .projmat import get_proj_matrix
 from .wavelets import findfreq
 
@@ -339,10 +339,10 @@ class SeisForwExpt:
             g.dsI[a] = float(ds[a])
         return g
 
-    def coef_arrays(self):
+    def coef_arrays(self, names=None):
         keep, arrs = [], []
         for name, a in self.coefs.items():
-            if coef_grid(self.physics, self.ndim, name) is None:
+            if coef_grid(self.physics, self.ndim, name) is None or (names is not None and name not in names):
                 continue
             a = np.asfortranarray(a, dtype=F32)
             keep.append(a)
@@ -573,23 +573,37 @@ class SeisForwExpt:
             out.append(slice(lo, hi))
         return tuple(out)
 
-    def _upload_coefs(self):
-        self._coefs = dependent(self.mod, self.dt32, self.order, self.ndim)
+    def _upload_coefs(self, changed=None):
+        """Re-derive ``dmod`` and refresh the device copies.  With ``changed == {"invK"}``
+        (acoustic) only dtK depends on it, so only dtK is derived and uploaded again."""
+        only_k = (changed is not None and set(changed) <= {"invK"} and self.physics == 0
+                  and self._coefs is not None)
+        if only_k:
+            self._coefs["invK"] = self.mod["invK"]
+            self._coefs["dtK"] = dtk_of(self.mod["invK"], self.dt32)
+        else:
+            self._coefs = dependent(self.mod, self.dt32, self.order, self.ndim)
         if self._model is not None:
-            keep, arrs = self.coef_arrays()
+            keep, arrs = self.coef_arrays(["dtK"] if only_k else None)
             self._ctx.check(self._ctx.lib.gpi_model_update(self._ctx.h, self._model, arrs, len(arrs)))
 
     def update_model(self, m: np.ndarray, mparams=None):
         """``update!(pa, m, mparams)`` (src/fdtd/medium.jl:31-52): the inner part of each
-        parameter becomes ``exp(m) * ref``; the padded part keeps its values."""
+        parameter becomes ``exp(m) * ref``; the padded part keeps its values.  Coefficients
+        of unchanged parameters are neither re-derived nor uploaded again."""
         mparams = list(mparams or self.mparams)
         chunks = np.split(np.asarray(m, dtype=np.float64), len(mparams))
+        changed = set()
         for x, name in zip(chunks, mparams):
             sl = self.view_inner(self.mod[name])
             shape = self.mod[name][sl].shape
             x32 = x.reshape(shape, order="F").astype(F32)
-            self.mod[name][sl] = np.exp(x32) * self.ref_mod[name]
-        self._upload_coefs()
+            new = np.exp(x32) * self.ref_mod[name]
+            if not np.array_equal(self.mod[name][sl], new):
+                self.mod[name][sl] = new
+                changed.add(name)
+        if changed or self._coefs is None:
+            self._upload_coefs(changed)
 
     def model_vector(self, mparams=None) -> np.ndarray:
         """``update!(m, pa, mparams)`` (src/fdtd/medium.jl:55-76): ``log(mod / ref)`` of the

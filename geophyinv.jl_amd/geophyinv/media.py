@@ -165,6 +165,11 @@ def coefficients(exmedium: Medium, dt32: np.float32, order: int) -> dict:
     return dependent(independent(exmedium), dt32, order, exmedium.ndim)
 
 
+def dtk_of(invK: np.ndarray, dt32: np.float32) -> np.ndarray:
+    """``dtK = dt * inv(invK)`` (src/fdtd/medium.jl:147-148)."""
+    return _f((F32(1) / invK) * dt32)
+
+
 def dependent(mod: dict, dt32: np.float32, order: int, nd: int) -> dict:
     """``update_dmod!`` (src/fdtd/medium.jl:143-186): ``mod`` plus the dt-scaled arrays."""
     h = order - 1
@@ -183,7 +188,7 @@ def dependent(mod: dict, dt32: np.float32, order: int, nd: int) -> dict:
         out["dtinvavzirho"] = _f(dt32 / _avg2(rho[0:nz - h, h:ny - h, h:nx - h],
                                              rho[1:nz - h + 1, h:ny - h, h:nx - h]))
     if "invK" in mod:
-        out["dtK"] = _f((F32(1) / mod["invK"]) * dt32)
+        out["dtK"] = dtk_of(mod["invK"], dt32)
         return out
     il = mod["invlambda"]
     im = mod["invmu"]
