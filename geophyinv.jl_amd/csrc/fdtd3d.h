@@ -56,6 +56,9 @@ struct StrParams3D {
 // coefficients it reads at row i are stored h floats past their allocation for the same
 // reason.  Returns the shift (0 for elastic).
 int vel3d_row_shift(const Model& m);
+// y rows per workgroup of the 3-D passes: acoustic GPI_3D_TY (4 or 8, default 4: on 440^3
+// the 64 x 8 tile ran 9 % slower), elastic 4
+int tile_rows_3d(const Model& m);
 int velocity_blocks_3d(const Model& m, int* gx, int* gy);
 int stress_blocks_3d(const Model& m, int* gx, int* gy);
 int velocity_pass_3d(const Model& m, const State& st, hipStream_t s, const PassExtras& X);

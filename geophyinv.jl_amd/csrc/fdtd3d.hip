@@ -100,7 +100,7 @@ __device__ __forceinline__ bool decode(int b, int gx, int gy, Idx3& p)
 // per-plane tile id bx + gx*(by + gy*k), as planned on the host.
 
 template <int O>
-__global__ __launch_bounds__(256) void vel3d_acoustic(VelParams3D P, PassExtras X)
+__global__ __launch_bounds__(512) void vel3d_acoustic(VelParams3D P, PassExtras X)
 {
     constexpr int H = O - 1;
     const int b = P.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
@@ -112,7 +112,8 @@ __global__ __launch_bounds__(256) void vel3d_acoustic(VelParams3D P, PassExtras 
         return;
     }
     const int bx = b % P.gx, r = b / P.gx, by = r % P.gy, ch = r / P.gy;
-    const int i = bx * 64 + (int)threadIdx.x - H, j = by * 4 + (int)threadIdx.y;   // rows shifted by h
+    const int i = bx * 64 + (int)threadIdx.x - H;   // rows shifted by h
+    const int j = by * (int)blockDim.y + (int)threadIdx.y;
     const int k0 = ch * P.kc, k1 = min(k0 + P.kc, nxw);
     const int s1 = (int)P.s1, s2 = (int)P.s2;
     const float* p = P.s[0];
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(256) void vel3d_acoustic(VelParams3D P, PassExtras 
 }
 
 template <int O>
-__global__ __launch_bounds__(256) void str3d_acoustic(StrParams3D P, PassExtras X)
+__global__ __launch_bounds__(512) void str3d_acoustic(StrParams3D P, PassExtras X)
 {
     const int b = P.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int nch = (P.nx + P.kc - 1) / P.kc;
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(256) void str3d_acoustic(StrParams3D P, PassExtras 
         return;
     }
     const int bx = b % P.gx, r = b / P.gx, by = r % P.gy, ch = r / P.gy;
-    const int i = bx * 64 + (int)threadIdx.x, j = by * 4 + (int)threadIdx.y;
+    const int i = bx * 64 + (int)threadIdx.x, j = by * (int)blockDim.y + (int)threadIdx.y;
     if (i >= P.nz || j >= P.ny) return;
     const int k0 = ch * P.kc, k1 = min(k0 + P.kc, P.nx);
     const int s1 = (int)P.s1, s2 = (int)P.s2;
@@ -483,10 +484,11 @@ void launch_vel3(const Model& m, hipStream_t s, VelParams3D P, const PassExtras&
 {
     constexpr int H = O - 1;
     P.gx = (P.nz - H + vel3d_row_shift(m) + 63) / 64;
-    P.gy = (P.ny - H + 3) / 4;
+    const int ty = tile_rows_3d(m);
+    P.gy = (P.ny - H + ty - 1) / ty;
     P.kc = m.g.physics == 0 ? march_planes(P.nx - H) : 1;
     P.remap = remap3();
-    dim3 grd(P.gx * P.gy * ((P.nx - H + P.kc - 1) / P.kc) + X.gather_blocks), blk(64, 4);
+    dim3 grd(P.gx * P.gy * ((P.nx - H + P.kc - 1) / P.kc) + X.gather_blocks), blk(64, ty);
     if (m.g.physics == 0) hipLaunchKernelGGL(vel3d_acoustic<O>, grd, blk, 0, s, P, X);
     else hipLaunchKernelGGL(vel3d_elastic<O>, grd, blk, 0, s, P, X);
 }
@@ -495,10 +497,11 @@ template <int O>
 void launch_str3(const Model& m, hipStream_t s, StrParams3D P, const PassExtras& X)
 {
     P.gx = (P.nz + 63) / 64;
-    P.gy = (P.ny + 3) / 4;
+    const int ty = tile_rows_3d(m);
+    P.gy = (P.ny + ty - 1) / ty;
     P.kc = m.g.physics == 0 ? march_planes(P.nx) : 1;
     P.remap = remap3();
-    dim3 grd(P.gx * P.gy * ((P.nx + P.kc - 1) / P.kc) + X.gather_blocks), blk(64, 4);
+    dim3 grd(P.gx * P.gy * ((P.nx + P.kc - 1) / P.kc) + X.gather_blocks), blk(64, ty);
     if (m.g.physics == 0) hipLaunchKernelGGL(str3d_acoustic<O>, grd, blk, 0, s, P, X);
     else hipLaunchKernelGGL(str3d_elastic<O>, grd, blk, 0, s, P, X);
 }
@@ -507,18 +510,28 @@ void launch_str3(const Model& m, hipStream_t s, StrParams3D P, const PassExtras&
 
 int vel3d_row_shift(const Model& m) { return m.g.physics == 0 ? m.h : 0; }
 
+int tile_rows_3d(const Model& m)
+{
+    static int ty = -1;
+    if (ty < 0) {
+        const char* e = getenv("GPI_3D_TY");
+        ty = e && atoi(e) == 8 ? 8 : 4;
+    }
+    return m.g.physics == 0 ? ty : 4;
+}
+
 int velocity_blocks_3d(const Model& m, int* gx, int* gy)
 {
     const int H = m.h;
     *gx = (int)((m.g.n[0] - H + vel3d_row_shift(m) + 63) / 64);
-    *gy = (int)((m.g.n[1] - H + 3) / 4);
+    *gy = (int)((m.g.n[1] - H + tile_rows_3d(m) - 1) / tile_rows_3d(m));
     return *gx * *gy * (int)(m.g.n[2] - H);
 }
 
 int stress_blocks_3d(const Model& m, int* gx, int* gy)
 {
     *gx = (int)((m.g.n[0] + 63) / 64);
-    *gy = (int)((m.g.n[1] + 3) / 4);
+    *gy = (int)((m.g.n[1] + tile_rows_3d(m) - 1) / tile_rows_3d(m));
     return *gx * *gy * (int)m.g.n[2];
 }
 

@@ -46,6 +46,7 @@ int plan_sources(gpi_ctx* c, const Model& m, Scratch& S, int nsrc, const gpi_poi
     const bool d3 = g.ndim == 3;
     const int vnb = d3 ? velocity_blocks_3d(m, &vgx, &vgy) : velocity_buckets_2d(m, &vgx);
     const int snb = d3 ? stress_blocks_3d(m, &sgx, &sgy) : stress_buckets_2d(m, &sgx);
+    const int ty3 = d3 ? tile_rows_3d(m) : 1;
     out.pts.assign(nsrc, DevPoints{});
     std::vector<int> kinds(nsrc);
     bool vel_fused = true, str_fused = true;
@@ -88,10 +89,10 @@ int plan_sources(gpi_ctx* c, const Model& m, Scratch& S, int nsrc, const gpi_poi
                 // muladd_with_density_v*! touches @inn(v) only (source.jl:166-177)
                 if (!(iz >= h && iz < n3[0] - h && ix >= h && ix < n3[2] - h)) continue;
                 if (d3 && !(iy >= h && iy < n3[1] - h)) continue;
-                blk = d3 ? (int)((iz - h + vel3d_row_shift(m)) / 64 + vgx * ((iy - h) / 4 + vgy * (ix - h)))
+                blk = d3 ? (int)((iz - h + vel3d_row_shift(m)) / 64 + vgx * ((iy - h) / ty3 + vgy * (ix - h)))
                          : (int)((iz - h) / 64 + vgx * (ix - h));
             } else {
-                blk = d3 ? (int)(iz / 64 + sgx * (iy / 4 + sgy * ix)) : (int)(iz / 64 + sgx * ix);
+                blk = d3 ? (int)(iz / 64 + sgx * (iy / ty3 + sgy * ix)) : (int)(iz / 64 + sgx * ix);
                 // a source on the free-surface row is applied after the surface condition
                 // by the stand-alone kernel (source.jl:61-119 runs after update_stress!)
                 if ((g.free_mask & GPI_FACE(0, 0)) && iz == 0) str_fused = false;
