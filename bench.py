@@ -39,17 +39,18 @@ def algorithmic_bytes_3d(n, npml):
 
 def algorithmic_bytes(nz, nx, npml, physics):
     """Compulsory FP32 bytes of one velocity pass and one stress pass (each array read once,
-    each output written once; derivatives never stored; CPML memory read+written in strips)."""
+    each output written once; derivatives never stored; CPML memory read+written in strips;
+    the dt-scaled coefficients are formed in the passes from rho, invlambda and invmu)."""
     N = nz * nx
     sx = strip_cells(N, nx, npml, 2)
     sz = strip_cells(N, nz, npml, 2)
     if physics == 1:
-        # velocity: r tauxx,tauzz,tauxz,vx,vz,2 coefs; w vx,vz = 9 floats; 2 psi per strip cell
-        vel = 36 * N + 16 * (sx + sz)
-        # stress: r vx,vz,tauxx,tauzz,tauxz,dtM,dtlambda,dtavmu; w 3 taus = 11 floats
-        stress = 44 * N + 16 * (sx + sz)
+        # velocity: r tauxx,tauzz,tauxz,vx,vz,rho; w vx,vz = 8 floats; 2 psi per strip cell
+        vel = 32 * N + 16 * (sx + sz)
+        # stress: r vx,vz,tauxx,tauzz,tauxz,invlambda,invmu; w 3 taus = 10 floats
+        stress = 40 * N + 16 * (sx + sz)
     else:
-        vel = 28 * N + 8 * (sx + sz)
+        vel = 24 * N + 8 * (sx + sz)
         stress = 20 * N + 8 * (sx + sz)
     return vel, stress
 

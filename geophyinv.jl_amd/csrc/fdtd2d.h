@@ -11,9 +11,7 @@ struct VelParams2D {
     const float* s2;   // elastic tauxz
     float* vx;
     float* vz;
-    const float* cx;   // dtinvavxirho
-    const float* cz;   // dtinvavzirho
-    const float* rho;  // density, for velocity sources
+    const float* rho;  // density: dtinvav{x,z}irho are formed from it, and velocity sources
     Pml qx0, qx1;      // x-derivative CPML: acoustic dpdx | elastic dtauxxdx, dtauxzdx
     Pml qz0, qz1;      // z-derivative CPML: acoustic dpdz | elastic dtauzzdz, dtauxzdz
     int nz, nx, ld, npml;
@@ -28,9 +26,9 @@ struct StrParams2D {
     float* s0;         // acoustic p | elastic tauxx
     float* s1;         // elastic tauzz
     float* s2;         // elastic tauxz
-    const float* c0;   // acoustic dtK | elastic dtM
-    const float* c1;   // elastic dtlambda
-    const float* c2;   // elastic dtavmu
+    const float* c0;   // acoustic dtK | elastic invlambda
+    const float* c1;   // elastic invmu: dtM, dtlambda and dtavmu are formed from c0, c1
+    float dt;
     Pml qx0, qx1;      // acoustic dvxdx | elastic dvxdx, dvzdx
     Pml qz0, qz1;      // acoustic dvzdz | elastic dvzdz, dvxdz
     int nz, nx, ld, npml;

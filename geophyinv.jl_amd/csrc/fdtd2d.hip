@@ -115,20 +115,23 @@ __device__ __forceinline__ void vel_ac_point(const VelParams2D& P, const PassExt
     const int ld = P.ld;
     const int bk = (i >> 6) + P.gx * j;
     const int o = (i + H) + ld * (j + H);
-    const int c = i + ld * j;
     const bool ux = i < P.nz - 2 * H && j < P.nx - H;   // vx on (nz-2h) x (nx-h)
     const bool uz = i < P.nz - H && j < P.nx - 2 * H;   // vz on (nz-h) x (nx-2h)
-    float px[O], pz[O], vxo = 0.0f, vzo = 0.0f, cxo = 0.0f, czo = 0.0f;
+    float px[O], pz[O], vxo = 0.0f, vzo = 0.0f, rx0 = 0.0f, rx1 = 0.0f, rz0 = 0.0f, rz1 = 0.0f;
     if (ux) {
         load_x<O>(px, P.s0, (i + H) + ld * j, ld);
         vxo = P.vx[o];
-        cxo = P.cx[c];
+        rx0 = P.rho[(i + H) + ld * j];
+        rx1 = P.rho[(i + H) + ld * (j + 1)];
     }
     if (uz) {
         load_z<O>(pz, P.s0, i + ld * (j + H));
         vzo = P.vz[o];
-        czo = P.cz[c];
+        rz0 = P.rho[i + ld * (j + H)];
+        rz1 = P.rho[(i + 1) + ld * (j + H)];
     }
+    const float cxo = dt_over_av2(P.dt, rx0, rx1);   // dtinvavxirho
+    const float czo = dt_over_av2(P.dt, rz0, rz1);   // dtinvavzirho
     // vx[inn] += dtinvavxirho * dpdx, dpdx = D_xi(p) * dxI
     if (ux) {
         float d = stencil<O>(px) * P.dxI;
@@ -188,22 +191,25 @@ __device__ __forceinline__ void vel_el_point(const VelParams2D& P, const PassExt
     const int ld = P.ld;
     const int bk = (i >> 6) + P.gx * j;
     const int o = (i + H) + ld * (j + H);
-    const int c = i + ld * j;
     const bool ux = i < P.nz - 2 * H && j < P.nx - H;   // vx on (nz-2h) x (nx-h)
     const bool uz = i < P.nz - H && j < P.nx - 2 * H;   // vz on (nz-h) x (nx-2h)
-    float a1[O], a2[O], b1[O], b2[O], vxo = 0.0f, vzo = 0.0f, cxo = 0.0f, czo = 0.0f;
+    float a1[O], a2[O], b1[O], b2[O], vxo = 0.0f, vzo = 0.0f, rx0 = 0.0f, rx1 = 0.0f, rz0 = 0.0f, rz1 = 0.0f;
     if (ux) {
         load_x<O>(a1, P.s0, (i + H) + ld * j, ld);   // tauxx along x
         load_z<O>(a2, P.s2, i + ld * j);             // tauxz along z
         vxo = P.vx[o];
-        cxo = P.cx[c];
+        rx0 = P.rho[(i + H) + ld * j];
+        rx1 = P.rho[(i + H) + ld * (j + 1)];
     }
     if (uz) {
         load_x<O>(b1, P.s2, i + ld * j, ld);         // tauxz along x
         load_z<O>(b2, P.s1, i + ld * (j + H));       // tauzz along z
         vzo = P.vz[o];
-        czo = P.cz[c];
+        rz0 = P.rho[i + ld * (j + H)];
+        rz1 = P.rho[(i + 1) + ld * (j + H)];
     }
+    const float cxo = dt_over_av2(P.dt, rx0, rx1);   // dtinvavxirho
+    const float czo = dt_over_av2(P.dt, rz0, rz1);   // dtinvavzirho
     // vx[inn] -= dtinvavxirho*(dtauxxdx + dtauxzdz)
     if (ux) {
         float d1 = stencil<O>(a1) * P.dxI;           // D_xi(tauxx)
@@ -241,12 +247,13 @@ __device__ __forceinline__ void str_el_point(const StrParams2D& P, const PassExt
     const bool un = i < P.nz && j < P.nx;             // tauxx, tauzz
     const bool us = i < P.nz - H && j < P.nx - H;     // tauxz on (nz-h) x (nx-h)
     float ax[O], az[O], bz[O], bx[O];
-    float m = 0.0f, l = 0.0f, s0o = 0.0f, s1o = 0.0f, s2o = 0.0f, c2o = 0.0f;
+    float il = 1.0f, im = 1.0f, s0o = 0.0f, s1o = 0.0f, s2o = 0.0f;
+    float m01 = 1.0f, m10 = 1.0f, m11 = 1.0f;
     if (un) {
         load_x<O>(ax, P.vx, o, ld);
         load_z<O>(az, P.vz, o);
-        m = P.c0[o];
-        l = P.c1[o];
+        il = P.c0[o];
+        im = P.c1[o];
         s0o = P.s0[o];
         s1o = P.s1[o];
     }
@@ -254,8 +261,13 @@ __device__ __forceinline__ void str_el_point(const StrParams2D& P, const PassExt
         load_z<O>(bz, P.vx, i + ld * (j + H));
         load_x<O>(bx, P.vz, (i + H) + ld * j, ld);
         s2o = P.s2[o];
-        c2o = P.c2[o];
+        m10 = P.c1[o + 1];
+        m01 = P.c1[o + ld];
+        m11 = P.c1[o + 1 + ld];
     }
+    const float m = dtm_of(il, im, P.dt);                     // dtM
+    const float l = (1.0f / il) * P.dt;                       // dtlambda
+    const float c2o = dt_over_av4(P.dt, im, m10, m01, m11);   // dtavmu
     if (un) {
         float dvx = stencil<O>(ax) * P.dxI;          // D_xa(vx)
         dvx = cpml_x(dvx, P.qx0, i, j, P.npml, P.pml_mask);
@@ -534,8 +546,6 @@ bool vel_params_2d(const Model& m, const State& st, VelParams2D* out)
     P.s2 = ac ? nullptr : st.field[GPI_TAUXZ];
     P.vx = st.field[GPI_VX];
     P.vz = st.field[GPI_VZ];
-    P.cx = m.coef[GPI_C_DTINVAVXIRHO];
-    P.cz = m.coef[GPI_C_DTINVAVZIRHO];
     P.rho = m.coef[GPI_C_RHO];
     P.qx0 = pml_of(m, st, ac ? GPI_DPDX : GPI_DTAUXXDX);
     P.qx1 = ac ? Pml{} : pml_of(m, st, GPI_DTAUXZDX);
@@ -551,7 +561,7 @@ bool vel_params_2d(const Model& m, const State& st, VelParams2D* out)
     P.dzI = m.g.dsI[0];
     P.dt = m.g.dt;
     const Pml* q[4] = {&P.qx0, &P.qx1, &P.qz0, &P.qz1};
-    if (!state_complete(m, q, 4) || !P.s0 || !P.vx || !P.vz || !P.cx || !P.cz || (!ac && (!P.s1 || !P.s2)))
+    if (!state_complete(m, q, 4) || !P.s0 || !P.vx || !P.vz || !P.rho || (!ac && (!P.s1 || !P.s2)))
         return false;
     *out = P;
     return true;
@@ -566,9 +576,9 @@ bool str_params_2d(const Model& m, const State& st, StrParams2D* out)
     P.s0 = st.field[ac ? GPI_P : GPI_TAUXX];
     P.s1 = ac ? nullptr : st.field[GPI_TAUZZ];
     P.s2 = ac ? nullptr : st.field[GPI_TAUXZ];
-    P.c0 = m.coef[ac ? GPI_C_DTK : GPI_C_DTM];
-    P.c1 = ac ? nullptr : m.coef[GPI_C_DTLAMBDA];
-    P.c2 = ac ? nullptr : m.coef[GPI_C_DTAVMU];
+    P.c0 = m.coef[ac ? GPI_C_DTK : GPI_C_INVLAMBDA];
+    P.c1 = ac ? nullptr : m.coef[GPI_C_INVMU];
+    P.dt = m.g.dt;
     P.qx0 = pml_of(m, st, GPI_DVXDX);
     P.qx1 = ac ? Pml{} : pml_of(m, st, GPI_DVZDX);
     P.qz0 = pml_of(m, st, GPI_DVZDZ);
@@ -582,7 +592,7 @@ bool str_params_2d(const Model& m, const State& st, StrParams2D* out)
     P.dxI = m.g.dsI[2];
     P.dzI = m.g.dsI[0];
     const Pml* q[4] = {&P.qx0, &P.qx1, &P.qz0, &P.qz1};
-    if (!state_complete(m, q, 4) || !P.s0 || !P.vx || !P.vz || !P.c0 || (!ac && (!P.s1 || !P.s2 || !P.c1 || !P.c2)))
+    if (!state_complete(m, q, 4) || !P.s0 || !P.vx || !P.vz || !P.c0 || (!ac && (!P.s1 || !P.s2 || !P.c1)))
         return false;
     *out = P;
     return true;

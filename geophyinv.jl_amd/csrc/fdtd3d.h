@@ -11,10 +11,10 @@ struct VelParams3D {
     float* vx;
     float* vy;
     float* vz;
-    const float* cx;   // dtinvavxirho
-    const float* cy;   // dtinvavyirho
-    const float* cz;   // dtinvavzirho
-    const float* rho;
+    const float* cx;   // acoustic: dtinvavxirho (the elastic pass forms it from rho)
+    const float* cy;   // acoustic: dtinvavyirho
+    const float* cz;   // acoustic: dtinvavzirho
+    const float* rho;  // density (velocity sources; elastic: the density coefficients)
     // CPML of the derivatives; acoustic: x[0] dpdx, y[0] dpdy, z[0] dpdz
     // elastic: vx uses x[0] dtauxxdx, y[0] dtauxydy, z[0] dtauxzdz
     //          vy uses x[1] dtauxydx, y[1] dtauyydy, z[1] dtauyzdz
@@ -35,8 +35,10 @@ struct StrParams3D {
     const float* vz;
     // acoustic: s[0] = p; elastic: tauxx, tauyy, tauzz, tauxy, tauxz, tauyz
     float* s[6];
-    // acoustic: c[0] = dtK; elastic: dtM, dtlambda, dtavxyimu, dtavxzimu, dtavyzimu
+    // acoustic: c[0] = dtK; elastic: c[0] = invlambda, c[1] = invmu (dtM, dtlambda and the
+    // averaged mu coefficients are formed from them in the pass)
     const float* c[5];
+    float dt;
     // acoustic: x[0] dvxdx, y[0] dvydy, z[0] dvzdz
     // elastic:  x: dvxdx, dvydx, dvzdx; y: dvydy, dvxdy, dvzdy; z: dvzdz, dvxdz, dvydz
     Pml qx[3], qy[3], qz[3];

@@ -96,8 +96,10 @@ bool required_coef(int ndim, int physics, int c)
     if (physics == 0)
         return c == GPI_C_DTINVAVXIRHO || c == GPI_C_DTINVAVZIRHO || c == GPI_C_DTK || c == GPI_C_RHO ||
                (ndim == 3 && c == GPI_C_DTINVAVYIRHO);
+    // the elastic passes form dtM, dtlambda and the averaged mu coefficients from invlambda
+    // and invmu in the kernels; the derived arrays stay for gpi_model_get_coef
     if (c == GPI_C_DTINVAVXIRHO || c == GPI_C_DTINVAVZIRHO || c == GPI_C_DTM || c == GPI_C_DTLAMBDA ||
-        c == GPI_C_RHO)
+        c == GPI_C_RHO || c == GPI_C_INVLAMBDA || c == GPI_C_INVMU)
         return true;
     if (ndim == 2) return c == GPI_C_DTAVMU;
     return c == GPI_C_DTINVAVYIRHO || c == GPI_C_DTAVXZIMU || c == GPI_C_DTAVXYIMU || c == GPI_C_DTAVYZIMU;

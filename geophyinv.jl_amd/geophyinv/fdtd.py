@@ -213,6 +213,7 @@ class SeisForwExpt:
         self.mparams = list(MPARAMS["acoustic" if physics == 0 else "elastic"])
         self._exmedium0 = self.exmedium
         self._mod = self._ref_mod = self._coefs = None
+        self._stale = set()   # mod entries changed since dmod was derived
         if coefficients == "host":
             self.coefs
         # ---- wavelets and frequency bounds (src/fdtd/source.jl:24-58, 192-246); with npw = 2
@@ -602,8 +603,10 @@ class SeisForwExpt:
             if not np.array_equal(self.mod[name][sl], new):
                 self.mod[name][sl] = new
                 changed.add(name)
+        changed |= self._stale
         if changed or self._coefs is None:
             self._upload_coefs(changed)
+        self._stale = set()
 
     def model_vector(self, mparams=None) -> np.ndarray:
         """``update!(m, pa, mparams)`` (src/fdtd/medium.jl:55-76): ``log(mod / ref)`` of the
@@ -614,7 +617,10 @@ class SeisForwExpt:
             r = self.ref_mod[name]
             lg = np.log(self.mod[name] * (F32(1) / r))
             out.append(np.asarray(lg[self.view_inner(lg)], np.float64).ravel(order="F"))
-            self.mod[name][...] = np.exp(lg) * r
+            rt = np.exp(lg) * r
+            if not np.array_equal(self.mod[name], rt):
+                self.mod[name][...] = rt
+                self._stale.add(name)   # dmod and the device copies follow at update_model
         return np.concatenate(out)
 
     def update_medium(self, medium: Medium):
