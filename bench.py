@@ -89,20 +89,27 @@ def cpu_baseline(pa, seconds):
                       f"first {steps} of {len(pa.tgrid)} time steps, {dt:.1f} s"}
 
 
-def run_plain(reqs, ctx, G):
+def _experiment(G, rq, **kw):
+    return G.SeisForwExpt(G.FdtdAcoustic(), medium=rq["medium"], tgrid=rq["tgrid"], ageom=rq["ageom"],
+                          srcwav=rq["srcwav"], **kw, **rq["kw"])
+
+
+def run_plain(reqs, ctx, G, prebuilt=False):
     """The drop-in path on the stream: one shot per call, in arrival order, each prepared on
-    the host as the reference does (coefficients included) and run by gpi_forward."""
+    the host as the reference does (coefficients included) and run by gpi_forward.  With
+    ``prebuilt`` the experiments (geometry weights, wavelets, CPML vectors) are made before
+    the stream starts; the coefficients are still derived and uploaded per shot."""
     import numpy as np
 
+    pas = [_experiment(G, rq, context=ctx, coefficients="host") for rq in reqs] if prebuilt else None
     t0 = time.perf_counter()
     lat, sums = [], []
-    for rq in reqs:
+    for k, rq in enumerate(reqs):
         t_arr = t0 + rq["arrival"]
         now = time.perf_counter()
         if now < t_arr:
             time.sleep(t_arr - now)
-        pa = G.SeisForwExpt(G.FdtdAcoustic(), medium=rq["medium"], tgrid=rq["tgrid"], ageom=rq["ageom"],
-                            srcwav=rq["srcwav"], context=ctx, coefficients="host", **rq["kw"])
+        pa = pas[k] if prebuilt else _experiment(G, rq, context=ctx, coefficients="host")
         recs, _ = pa.run_shot(0)
         pa.free()
         lat.append(time.perf_counter() - t_arr)
@@ -111,22 +118,25 @@ def run_plain(reqs, ctx, G):
     return span, np.array(lat), sums
 
 
-def run_served(reqs, G, opts):
+def run_served(reqs, G, opts, prebuilt=False):
     """The same stream through the serving scheduler: each shot is prepared on the host
-    (weights, CPML vectors; coefficients on the device) and submitted at its arrival."""
+    (weights, CPML vectors; coefficients on the device) and submitted at its arrival; with
+    ``prebuilt`` the host preparation happens before the stream starts, as in run_plain."""
     import numpy as np
     from geophyinv.serve import Server
 
+    pas = [_experiment(G, rq, coefficients="device") for rq in reqs] if prebuilt else None
     with Server(0, 2, **opts) as srv:
         t0 = time.perf_counter()
         tickets = []
-        for rq in reqs:
+        for k, rq in enumerate(reqs):
             t_arr = t0 + rq["arrival"]
             now = time.perf_counter()
             if now < t_arr:
                 time.sleep(t_arr - now)
-            pa = G.SeisForwExpt(G.FdtdAcoustic(), medium=rq["medium"], tgrid=rq["tgrid"], ageom=rq["ageom"],
-                                srcwav=rq["srcwav"], coefficients="device", **rq["kw"])
+            pa = pas[k] if prebuilt else _experiment(G, rq, coefficients="device")
+            if prebuilt:
+                pas[k] = None
             lag = time.perf_counter() - t_arr
             tickets.append((srv.submit(pa, 0), lag))
             del pa
@@ -140,24 +150,34 @@ def run_served(reqs, G, opts):
     return span, np.array(lat), sums, st
 
 
-def serving_section(nshots, opts, G, _lib):
+def serving_section(nshots, opts, G, _lib, side=(256, 2048), steps=(1000, 6000), compare_direct=False,
+                    rate=200.0, prebuilt=False):
     from geophyinv import gallery
     import numpy as np
 
     t = time.perf_counter()
-    reqs = gallery.serving_stream(nshots=nshots)
+    reqs = gallery.serving_stream(nshots=nshots, side=tuple(side), steps=tuple(steps), rate=rate)
     gen_s = time.perf_counter() - t
     cells = float(sum((r["side"] + 2 * r["kw"]["npml"]) ** 2 * r["nt"] for r in reqs))
     ctx = _lib.Context(0)
-    p_span, p_lat, p_sums = run_plain(reqs, ctx, G)
+    p_span, p_lat, p_sums = run_plain(reqs, ctx, G, prebuilt)
     ctx.close()
-    s_span, s_lat, s_sums, st = run_served(reqs, G, opts)
+    s_span, s_lat, s_sums, st = run_served(reqs, G, opts, prebuilt)
     same = sum(1 for a, b in zip(p_sums, s_sums) if a == b)
     pct = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 1)
+    direct = None
+    if compare_direct:   # the same stream with every pass launched directly (no graphs)
+        d_span, d_lat, d_sums, d_st = run_served(reqs, G, dict(opts, graphs=False), prebuilt)
+        direct = {"shots_per_s": round(nshots / d_span, 3), "span_s": round(d_span, 2),
+                  "p50_ms": pct(d_lat, 50), "p99_ms": pct(d_lat, 99),
+                  "device_busy_s": round(d_st["busy_ms"] * 1e-3, 2),
+                  "records_identical": f"{sum(1 for a, b in zip(p_sums, d_sums) if a == b)}/{nshots}"}
     return {
-        "workload": f"{nshots} 2-D acoustic shots (order 2), Poisson arrivals at 200/s, side U{{256..2048}} at "
-                    "5 m + 20-cell CPML, nt U{1000..6000}, 10-30 Hz Ricker on p, 64-512 surface receivers "
-                    "(p, vz), layered medium + Gaussian anomaly; seed 3003",
+        "workload": f"{nshots} 2-D acoustic shots (order 2), Poisson arrivals at {rate:g}/s, side "
+                    f"U{{{side[0]}..{side[1]}}} at 5 m + 20-cell CPML, nt U{{{steps[0]}..{steps[1]}}}, 10-30 Hz "
+                    "Ricker on p, 64-512 surface receivers (p, vz), layered medium + Gaussian anomaly; seed 3003"
+                    + ("; host preparation of each shot before the stream starts" if prebuilt else ""),
+        "arrivals_span_s": round(reqs[-1]["arrival"], 3),
         "shots": nshots,
         "gpt_updates": cells,
         "plain": {"shots_per_s": round(nshots / p_span, 3), "span_s": round(p_span, 2),
@@ -167,7 +187,9 @@ def serving_section(nshots, opts, G, _lib):
                    "p50_ms": pct(s_lat, 50), "p99_ms": pct(s_lat, 99),
                    "gpt_per_s": round(cells / s_span / 1e9, 2), "device_busy_s": round(st["busy_ms"] * 1e-3, 2),
                    "mean_batch": round(st["mean_batch"], 2), "tile": [st["tile_ty"], st["tile_rx"]],
-                   "options": opts},
+                   "graph_chunks": st["graph_chunks"], "graph_captures": st["graph_captures"],
+                   "batched_steps": st["steps"], "options": opts},
+        "served_direct_launch": direct,
         "speedup_shots_per_s": round(p_span / s_span, 3),
         "records_identical": f"{same}/{nshots}",
         "generation_s": round(gen_s, 1),
@@ -292,7 +314,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt", type=int, default=0, help="time steps (default: the workload's own)")
-    ap.add_argument("--workload", choices=["elastic2d", "acoustic3d", "gradient3d"], default="elastic2d")
+    ap.add_argument("--workload", choices=["elastic2d", "acoustic3d", "gradient3d", "serve"], default="elastic2d")
     ap.add_argument("--checkpoint", type=int, default=100, help="gradient3d: checkpoint interval")
     ap.add_argument("--fd-cells", type=int, default=0, help="gradient3d: finite-difference checks")
     ap.add_argument("--fd-target", type=float, default=1e-3, help="gradient3d: relative misfit change per FD step")
@@ -305,6 +327,12 @@ def main():
     ap.add_argument("--serve", action="store_true")
     ap.add_argument("--max-shots", type=int, default=64)
     ap.add_argument("--chunk-steps", type=int, default=32)
+    ap.add_argument("--serve-side", type=int, nargs=2, default=[256, 2048], help="serving: grid side range")
+    ap.add_argument("--serve-steps", type=int, nargs=2, default=[1000, 6000], help="serving: time step range")
+    ap.add_argument("--serve-direct", action="store_true", help="serving: also run without graph replay")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="serving: Poisson arrival rate (1/s)")
+    ap.add_argument("--serve-prebuilt", action="store_true",
+                    help="serving: prepare every shot on the host before the stream starts (both paths)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the single-GPU path only "
@@ -316,6 +344,18 @@ def main():
     import geophyinv as G
     from geophyinv import _lib, gallery
 
+    if args.workload == "serve":
+        opts = dict(max_shots=args.max_shots, chunk_steps=args.chunk_steps, autotune=True)
+        sec = serving_section(args.serve_shots, opts, G, _lib, args.serve_side, args.serve_steps, args.serve_direct,
+                              args.serve_rate, args.serve_prebuilt)
+        print(json.dumps({
+            "metric": "shots/s (2-D acoustic serving stream, continuous batching)",
+            "value": sec["served"]["shots_per_s"], "unit": "shots/s", "n_gpus": 1, "steps": args.serve_shots,
+            "warmup": 0, "ms_per_step": round(1e3 / max(sec["served"]["shots_per_s"], 1e-9), 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": sec["workload"], "max_shots": args.max_shots, "chunk_steps": args.chunk_steps},
+            "serving": sec}), flush=True)
+        return
     if args.workload == "gradient3d":
         print(json.dumps(run_gradient3d(args, G, _lib, gallery)), flush=True)
         return

@@ -69,7 +69,9 @@ struct BatchShot2D {
 // One pass of every shot of a device table: first[s] = first workgroup of shot s,
 // first[n] = nblocks; all shots share physics, order and tile shape.
 void batch_pass_2d(int physics, int order, const TileShape& t, bool velocity, const BatchShot2D* D,
-                   const int* first, int n, int nblocks, int64_t step, hipStream_t s);
+                   const int* first, int n, int nblocks, const int64_t* step0, int koff, hipStream_t s);
+// *step0 = step, stream-ordered (a one-thread kernel: no memset node in a captured chunk).
+void batch_set_step(int64_t* step0, int64_t step, hipStream_t s);
 
 // The pass launchers return -1 without launching when the state lacks an array the pass
 // writes (a wavefield, or the CPML memory of an active derivative).

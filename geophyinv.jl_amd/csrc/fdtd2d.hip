@@ -419,7 +419,9 @@ void launch_str(const Model& m, hipStream_t s, StrParams2D P, const PassExtras& 
 // workgroups back to back; first[s] is shot s's first workgroup (first[n] the total).
 // Each shot advances at its own time step it = step - start: shots past their last step
 // idle, and at it == nt a shot takes the trailing velocity half step of
-// src/fdtd/propagate.jl:254-255 (no sources, no receivers).
+// src/fdtd/propagate.jl:254-255 (no sources, no receivers).  The batch's step is read from
+// device memory (*step0, set by batch_set_step before a chunk) plus the launch's offset
+// koff in the chunk, so one captured graph of a chunk's launches replays at any step.
 // ---------------------------------------------------------------------------------------
 
 __device__ __forceinline__ int shot_of_block(const int* __restrict__ first, int n, int b)
@@ -435,8 +437,9 @@ __device__ __forceinline__ int shot_of_block(const int* __restrict__ first, int 
 
 template <int O, int PHYS, int TY, int RX>
 __global__ __launch_bounds__(64 * TY) void vel2d_batch(const BatchShot2D* __restrict__ D, const int* __restrict__ first,
-                                                       int n, int64_t step)
+                                                       int n, const int64_t* __restrict__ step0, int koff)
 {
+    const int64_t step = *step0 + koff;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int s = shot_of_block(first, n, b);
     const BatchShot2D& S = D[s];
@@ -453,8 +456,9 @@ __global__ __launch_bounds__(64 * TY) void vel2d_batch(const BatchShot2D* __rest
 
 template <int O, int PHYS, int TY, int RX>
 __global__ __launch_bounds__(64 * TY) void str2d_batch(const BatchShot2D* __restrict__ D, const int* __restrict__ first,
-                                                       int n, int64_t step)
+                                                       int n, const int64_t* __restrict__ step0, int koff)
 {
+    const int64_t step = *step0 + koff;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int s = shot_of_block(first, n, b);
     const BatchShot2D& S = D[s];
@@ -469,23 +473,28 @@ __global__ __launch_bounds__(64 * TY) void str2d_batch(const BatchShot2D* __rest
     str_tile<O, PHYS, TY, RX>(S.sp, S.xs, it, S.xs.ninj, lb);
 }
 
+__global__ void batch_set_step_kernel(int64_t* step0, int64_t step)
+{
+    if (threadIdx.x == 0) *step0 = step;
+}
+
 template <int O, int PHYS, int TY, int RX>
-void launch_batch_t(bool velocity, const BatchShot2D* D, const int* first, int n, int nblocks, int64_t step,
-                    hipStream_t s)
+void launch_batch_t(bool velocity, const BatchShot2D* D, const int* first, int n, int nblocks,
+                    const int64_t* step0, int koff, hipStream_t s)
 {
     dim3 grd(nblocks), blk(64, TY);
-    if (velocity) hipLaunchKernelGGL((vel2d_batch<O, PHYS, TY, RX>), grd, blk, 0, s, D, first, n, step);
-    else hipLaunchKernelGGL((str2d_batch<O, PHYS, TY, RX>), grd, blk, 0, s, D, first, n, step);
+    if (velocity) hipLaunchKernelGGL((vel2d_batch<O, PHYS, TY, RX>), grd, blk, 0, s, D, first, n, step0, koff);
+    else hipLaunchKernelGGL((str2d_batch<O, PHYS, TY, RX>), grd, blk, 0, s, D, first, n, step0, koff);
 }
 
 template <int O, int PHYS>
 void launch_batch_o(const TileShape& t, bool velocity, const BatchShot2D* D, const int* first, int n, int nblocks,
-                    int64_t step, hipStream_t s)
+                    const int64_t* step0, int koff, hipStream_t s)
 {
-    if (t.ty == 4 && t.rx == 1) launch_batch_t<O, PHYS, 4, 1>(velocity, D, first, n, nblocks, step, s);
-    else if (t.ty == 8 && t.rx == 1) launch_batch_t<O, PHYS, 8, 1>(velocity, D, first, n, nblocks, step, s);
-    else if (t.ty == 2 && t.rx == 2) launch_batch_t<O, PHYS, 2, 2>(velocity, D, first, n, nblocks, step, s);
-    else launch_batch_t<O, PHYS, 4, 2>(velocity, D, first, n, nblocks, step, s);
+    if (t.ty == 4 && t.rx == 1) launch_batch_t<O, PHYS, 4, 1>(velocity, D, first, n, nblocks, step0, koff, s);
+    else if (t.ty == 8 && t.rx == 1) launch_batch_t<O, PHYS, 8, 1>(velocity, D, first, n, nblocks, step0, koff, s);
+    else if (t.ty == 2 && t.rx == 2) launch_batch_t<O, PHYS, 2, 2>(velocity, D, first, n, nblocks, step0, koff, s);
+    else launch_batch_t<O, PHYS, 4, 2>(velocity, D, first, n, nblocks, step0, koff, s);
 }
 
 }  // namespace
@@ -517,24 +526,29 @@ void stress_grid_2d(const Model& m, const TileShape& t, int* gx, int* gy)
 }
 
 void batch_pass_2d(int physics, int order, const TileShape& t, bool velocity, const BatchShot2D* D, const int* first,
-                   int n, int nblocks, int64_t step, hipStream_t s)
+                   int n, int nblocks, const int64_t* step0, int koff, hipStream_t s)
 {
     if (nblocks <= 0) return;
     if (physics == 0) {
         switch (order) {
-        case 2: launch_batch_o<2, 0>(t, velocity, D, first, n, nblocks, step, s); break;
-        case 4: launch_batch_o<4, 0>(t, velocity, D, first, n, nblocks, step, s); break;
-        case 6: launch_batch_o<6, 0>(t, velocity, D, first, n, nblocks, step, s); break;
-        default: launch_batch_o<8, 0>(t, velocity, D, first, n, nblocks, step, s); break;
+        case 2: launch_batch_o<2, 0>(t, velocity, D, first, n, nblocks, step0, koff, s); break;
+        case 4: launch_batch_o<4, 0>(t, velocity, D, first, n, nblocks, step0, koff, s); break;
+        case 6: launch_batch_o<6, 0>(t, velocity, D, first, n, nblocks, step0, koff, s); break;
+        default: launch_batch_o<8, 0>(t, velocity, D, first, n, nblocks, step0, koff, s); break;
         }
     } else {
         switch (order) {
-        case 2: launch_batch_o<2, 1>(t, velocity, D, first, n, nblocks, step, s); break;
-        case 4: launch_batch_o<4, 1>(t, velocity, D, first, n, nblocks, step, s); break;
-        case 6: launch_batch_o<6, 1>(t, velocity, D, first, n, nblocks, step, s); break;
-        default: launch_batch_o<8, 1>(t, velocity, D, first, n, nblocks, step, s); break;
+        case 2: launch_batch_o<2, 1>(t, velocity, D, first, n, nblocks, step0, koff, s); break;
+        case 4: launch_batch_o<4, 1>(t, velocity, D, first, n, nblocks, step0, koff, s); break;
+        case 6: launch_batch_o<6, 1>(t, velocity, D, first, n, nblocks, step0, koff, s); break;
+        default: launch_batch_o<8, 1>(t, velocity, D, first, n, nblocks, step0, koff, s); break;
         }
     }
+}
+
+void batch_set_step(int64_t* step0, int64_t step, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_set_step_kernel, dim3(1), dim3(64), 0, s, step0, step);
 }
 
 bool vel_params_2d(const Model& m, const State& st, VelParams2D* out)

@@ -7,9 +7,13 @@
 // kernels), each shot at its own step.  Between chunks of steps, finished shots leave
 // (records copied to pinned memory) and waiting shots enter as slots and the HBM budget
 // allow.  Device memory comes from hipMalloc; the memory of departed shots is released
-// only after both streams have drained, so no queued launch can reference freed memory.  At most two chunks are in flight, so
-// admission and copy-back overlap the device work.  The workgroup shape of the batched
-// kernels is chosen per batch size by timing each candidate on live chunks.
+// only after both streams have drained, so no queued launch can reference freed memory.
+// At most two chunks are in flight, so admission and copy-back overlap the device work.
+// The workgroup shape of the batched kernels is chosen per batch size by timing each
+// candidate on live chunks.  A full chunk whose batch and shape are those of the chunk
+// before it is one hipGraphLaunch of a captured graph of its 2 x chunk_steps launches
+// (cached per batch layout, least recently used evicted); the kernels read the batch step
+// from device memory, set before each chunk, so a graph replays at any step.
 //
 // No reference counterpart: the reference propagates supersources one after another on
 // each worker (src/fdtd/propagate.jl:100-106, :138-261).
@@ -129,6 +133,7 @@ struct gpi_server {
     std::string err;
     // statistics (guarded by mu)
     int64_t submitted = 0, completed = 0, failed = 0, steps = 0, launches = 0;
+    int64_t graph_chunks = 0, graph_captures = 0;
     double busy_ms = 0.0, batch_steps = 0.0;
     std::vector<double> latency;
     std::map<uint64_t, double> lat_of;   // per finished ticket
@@ -146,6 +151,18 @@ struct gpi_server {
     hipEvent_t adm_ev = nullptr;
     hipEvent_t ck_ev[2][2] = {};   // [chunk parity][start, end]
     std::map<int, int> best_shape; // batch-size class -> shape index
+    // captured chunks: one per batch layout (shape, shots, workgroups of both passes)
+    struct ChunkGraph {
+        int shape, ns, nbv, nbs;
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    static constexpr size_t kMaxGraphs = 8;
+    std::vector<ChunkGraph> graphs;
+    std::vector<hipGraphExec_t> graph_grave;   // evicted; destroyed once the streams drain
+    uint64_t graph_clock = 0;
+    hipStream_t cap = nullptr;                 // capture stream (never runs work)
+    int64_t* dstep = nullptr;                  // the batch step the kernels read
 
     void run();
     void collect();
@@ -154,16 +171,68 @@ struct gpi_server {
     void finalize(Slot* s, int code);
     void fail_req(uint64_t ticket, int code, const std::string& msg);
     int upload_table(const TileShape& t, int* nbv, int* nbs);
+    int chunk_graph(int shape, int ns, int nbv, int nbs, hipGraphExec_t* out);
+    void launch_chunk(const TileShape& t, int ns, int nbv, int nbs, int64_t K, hipStream_t st);
 };
 
 void gpi_server::collect()
 {
-    if (grave.empty()) return;
+    if (grave.empty() && graph_grave.empty()) return;
     (void)hipStreamSynchronize(ac->stream);
     (void)hipStreamSynchronize(cc->stream);
     for (void* p : grave) (void)hipFree(p);
     grave.clear();
     grave_bytes = 0;
+    for (hipGraphExec_t g : graph_grave) (void)hipGraphExecDestroy(g);
+    graph_grave.clear();
+}
+
+// The 2 x chunk_steps launches of one full chunk, kernel offsets 0 .. chunk_steps - 1.
+void gpi_server::launch_chunk(const TileShape& t, int ns, int nbv, int nbs, int64_t K, hipStream_t st)
+{
+    for (int64_t k = 0; k < K; ++k) {
+        batch_pass_2d(o.physics, o.order, t, true, dtab, dfirst, ns, nbv, dstep, (int)k, st);
+        batch_pass_2d(o.physics, o.order, t, false, dtab, dfirst + o.max_shots + 1, ns, nbs, dstep, (int)k, st);
+    }
+}
+
+// The executable graph of a full chunk for this batch layout, captured on first use.  The
+// table contents may change between replays (they are copied before the chunk that needs
+// them); the table and step addresses the launches read are fixed for the server's life.
+int gpi_server::chunk_graph(int shape, int ns, int nbv, int nbs, hipGraphExec_t* out)
+{
+    for (ChunkGraph& g : graphs)
+        if (g.shape == shape && g.ns == ns && g.nbv == nbv && g.nbs == nbs) {
+            g.used = ++graph_clock;
+            *out = g.exec;
+            return GPI_OK;
+        }
+    hipGraph_t gr = nullptr;
+    HIPTRY(cc, hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed));
+    launch_chunk(TILE_SHAPES[shape], ns, nbv, nbs, o.chunk_steps, cap);
+    const hipError_t le = hipGetLastError();
+    const hipError_t ee = hipStreamEndCapture(cap, &gr);
+    if (le != hipSuccess || ee != hipSuccess) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return fail(cc, GPI_EHIP, std::string("chunk capture: ") + hipGetErrorString(le != hipSuccess ? le : ee));
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (ie != hipSuccess) return fail(cc, GPI_EHIP, std::string("chunk graph instantiate: ") + hipGetErrorString(ie));
+    if (graphs.size() >= kMaxGraphs) {
+        auto lru = std::min_element(graphs.begin(), graphs.end(),
+                                    [](const ChunkGraph& a, const ChunkGraph& b) { return a.used < b.used; });
+        graph_grave.push_back(lru->exec);   // may still be queued on the compute stream
+        graphs.erase(lru);
+    }
+    graphs.push_back(ChunkGraph{shape, ns, nbv, nbs, ex, ++graph_clock});
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        ++graph_captures;
+    }
+    *out = ex;
+    return GPI_OK;
 }
 
 void gpi_server::fail_req(uint64_t ticket, int code, const std::string& msg)
@@ -366,7 +435,7 @@ void gpi_server::run()
             }
         }
         // memory of departed shots holds back admission: drain and release it
-        if ((blocked && grave_bytes > 0) || grave_bytes > budget / 8) collect();
+        if ((blocked && grave_bytes > 0) || grave_bytes > budget / 8 || graph_grave.size() >= kMaxGraphs) collect();
         bool admitted = false;
         for (auto& r : take) {
             const uint64_t ticket = r->ticket;
@@ -428,6 +497,7 @@ void gpi_server::run()
             }
         }
         const TileShape t = TILE_SHAPES[shape];
+        const bool fresh = dirty;   // a new batch layout or shape: launched directly once
         if (dirty) {
             if (upload_table(t, &nbv, &nbs)) {
                 shot_fail_all(GPI_EHIP, cc->err);
@@ -447,10 +517,22 @@ void gpi_server::run()
         }
         (void)hipEventRecord(E[0], cc->stream);
         const int ns = (int)active.size();
-        for (int64_t k = 0; k < K; ++k) {
-            batch_pass_2d(o.physics, o.order, t, true, dtab, dfirst, ns, nbv, step + k, cc->stream);
-            batch_pass_2d(o.physics, o.order, t, false, dtab, dfirst + o.max_shots + 1, ns, nbs, step + k,
-                          cc->stream);
+        batch_set_step(dstep, step, cc->stream);
+        bool graphed = false;
+        if (!o.direct_launch && !fresh && K == o.chunk_steps) {
+            hipGraphExec_t ex = nullptr;
+            if (chunk_graph(shape, ns, nbv, nbs, &ex)) {
+                shot_fail_all(GPI_EHIP, cc->err);
+                continue;
+            }
+            const hipError_t ge = hipGraphLaunch(ex, cc->stream);
+            if (ge != hipSuccess) {
+                shot_fail_all(GPI_EHIP, std::string("chunk graph launch: ") + hipGetErrorString(ge));
+                continue;
+            }
+            graphed = true;
+        } else {
+            launch_chunk(t, ns, nbv, nbs, K, cc->stream);
         }
         (void)hipEventRecord(E[1], cc->stream);
         const hipError_t le = hipGetLastError();
@@ -464,6 +546,7 @@ void gpi_server::run()
             std::lock_guard<std::mutex> lk(mu);
             steps += K;
             launches += 2 * K;
+            graph_chunks += graphed ? 1 : 0;
             batch_steps += (double)ns * (double)K;
             last_ty = t.ty;
             last_rx = t.rx;
@@ -513,7 +596,8 @@ int gpi_server_create(int device, const gpi_server_opts* opts, gpi_server** out)
     *out = nullptr;
     const gpi_server_opts& o = *opts;
     if ((o.physics != 0 && o.physics != 1) || (o.order != 2 && o.order != 4 && o.order != 6 && o.order != 8) ||
-        o.max_shots < 0 || o.max_shots > 4096 || o.chunk_steps < 0 || o.tile < 0 || o.tile >= NTILE_SHAPES)
+        o.max_shots < 0 || o.max_shots > 4096 || o.chunk_steps < 0 || o.chunk_steps > 4096 || o.tile < 0 ||
+        o.tile >= NTILE_SHAPES || o.direct_launch < 0 || o.direct_launch > 1 || o.reserved != 0)
         return GPI_EINVAL;
     gpi_server* v = new gpi_server();
     v->o = o;
@@ -538,6 +622,8 @@ int gpi_server_create(int device, const gpi_server_opts* opts, gpi_server** out)
     const size_t tb = sizeof(BatchShot2D) * v->o.max_shots + sizeof(int) * 2 * (v->o.max_shots + 1);
     bool ok = hipMalloc((void**)&v->dtab, sizeof(BatchShot2D) * v->o.max_shots) == hipSuccess &&
               hipMalloc((void**)&v->dfirst, sizeof(int) * 2 * (v->o.max_shots + 1)) == hipSuccess &&
+              hipMalloc((void**)&v->dstep, sizeof(int64_t)) == hipSuccess &&
+              hipStreamCreateWithFlags(&v->cap, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreateWithFlags(&v->adm_ev, hipEventDisableTiming) == hipSuccess;
     for (int k = 0; k < 2 && ok; ++k) {
         ok = hipHostMalloc((void**)&v->htab[k], tb, hipHostMallocDefault) == hipSuccess &&
@@ -569,8 +655,12 @@ void gpi_server_destroy(gpi_server* v)
         (void)hipSetDevice(v->cc->device);
         (void)hipStreamSynchronize(v->cc->stream);
     }
+    for (auto& g : v->graphs) (void)hipGraphExecDestroy(g.exec);
+    for (hipGraphExec_t g : v->graph_grave) (void)hipGraphExecDestroy(g);
+    if (v->cap) (void)hipStreamDestroy(v->cap);
     if (v->dtab) (void)hipFree(v->dtab);
     if (v->dfirst) (void)hipFree(v->dfirst);
+    if (v->dstep) (void)hipFree(v->dstep);
     for (int k = 0; k < 2; ++k) {
         if (v->htab[k]) (void)hipHostFree(v->htab[k]);
         if (v->tab_ev[k]) (void)hipEventDestroy(v->tab_ev[k]);
@@ -725,6 +815,8 @@ int gpi_server_get_stats(gpi_server* v, gpi_server_stats* out)
     s.queued = (int64_t)v->queue.size();
     s.steps = v->steps;
     s.launches = v->launches;
+    s.graph_chunks = v->graph_chunks;
+    s.graph_captures = v->graph_captures;
     s.busy_ms = v->busy_ms;
     s.mean_batch = v->steps ? v->batch_steps / (double)v->steps : 0.0;
     s.tile_ty = v->last_ty;

@@ -13,6 +13,7 @@ from .fields import FIELD_ID, FIELDS, field_shape, get_mgrid
 from .fwi import SeisInvExpt
 from .grid import Grid
 from .media import AcousticMedium, ElasticMedium, Medium, padarray
+from .chainrules import rrule
 from .wavelets import findfreq, ricker
 
 AGeom = list
@@ -37,5 +38,5 @@ def gradient(*args, **kw):
 __all__ = [
     "AGeom", "AGeomss", "AcousticMedium", "Born", "ElasticMedium", "LinearMap", "FIELDS", "FIELD_ID", "FdtdAcoustic",
     "FdtdElastic", "Grid", "L2DistLoss", "Medium", "Records", "Recs", "SeisForwExpt", "SeisInvExpt", "SrcWav", "Srcs",
-    "field_shape", "findfreq", "get_mgrid", "gradient", "lossvalue", "padarray", "ricker", "update",
+    "field_shape", "findfreq", "get_mgrid", "gradient", "lossvalue", "padarray", "ricker", "rrule", "update",
 ]

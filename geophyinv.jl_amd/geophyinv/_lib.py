@@ -71,7 +71,7 @@ class gpi_medium(C.Structure):
 class gpi_server_opts(C.Structure):
     _fields_ = [("physics", C.c_int32), ("order", C.c_int32), ("max_shots", C.c_int32),
                 ("chunk_steps", C.c_int32), ("autotune", C.c_int32), ("tile", C.c_int32),
-                ("hbm_budget_bytes", C.c_uint64)]
+                ("direct_launch", C.c_int32), ("reserved", C.c_int32), ("hbm_budget_bytes", C.c_uint64)]
 
 
 class gpi_request(C.Structure):
@@ -84,7 +84,8 @@ class gpi_server_stats(C.Structure):
                 ("active", C.c_int64), ("queued", C.c_int64), ("steps", C.c_int64), ("launches", C.c_int64),
                 ("busy_ms", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
                 ("mean_batch", C.c_double), ("tile_ty", C.c_int32), ("tile_rx", C.c_int32),
-                ("hbm_used", C.c_uint64), ("hbm_budget", C.c_uint64)]
+                ("hbm_used", C.c_uint64), ("hbm_budget", C.c_uint64), ("graph_chunks", C.c_int64),
+                ("graph_captures", C.c_int64)]
 
 
 class gpi_timing(C.Structure):

@@ -237,6 +237,7 @@ class SeisForwExpt:
         self._model = None
         self._stores = {}
         self.timing = None
+        self.generation = 0   # bumped by every run and every change of model, sources or geometry
 
     # ------------------------------------------------------------------ host preparation
     @property
@@ -539,6 +540,7 @@ class SeisForwExpt:
         ``initialize!(pa.c)`` does, and stacks the per-supersource gradients into
         ``pa.gradients`` in supersource order (sum_grads!, gradient.jl:2-12)."""
         mode = self.attrib_mod.mode
+        self.generation += 1
         for d in self.data:
             d.fill(0.0)
         for g in self.gradients.values():
@@ -603,6 +605,8 @@ class SeisForwExpt:
             if not np.array_equal(self.mod[name][sl], new):
                 self.mod[name][sl] = new
                 changed.add(name)
+        if changed:
+            self.generation += 1
         changed |= self._stale
         if changed or self._coefs is None:
             self._upload_coefs(changed)
@@ -627,6 +631,7 @@ class SeisForwExpt:
         """``update!(pa, medium)`` without reallocation (src/fdtd/medium.jl:128-140)."""
         if type(medium) is not type(self.medium) or [g.n for g in medium.grid] != [g.n for g in self.medium.grid]:
             raise ValueError("medium must match the experiment's medium type and grid")
+        self.generation += 1
         self.medium = medium.copy()
         self.exmedium = padarray(self.medium, self.npml, self.pml_faces)
         self._mod = None
@@ -685,6 +690,7 @@ class SeisForwExpt:
 
         ``srcwav`` is a SrcWav (one Srcs per supersource) for the first wavefield, or, with
         npw = 2, a list [SrcWav, SrcWav] for both; ``src_types`` has one entry per wavefield."""
+        self.generation += 1
         srcwav = list(srcwav) if isinstance(srcwav, (list, tuple)) else [srcwav]
         per_pw = bool(srcwav) and isinstance(srcwav[0], (list, tuple))
         sw = [list(x) for x in srcwav] if per_pw else [srcwav]
@@ -701,6 +707,7 @@ class SeisForwExpt:
 
     def update_ageom(self, ageom):
         """``update!(pa, ageom)`` (src/fdtd/ageom.jl:58-98)."""
+        self.generation += 1
         ageom = list(ageom) if isinstance(ageom, (list, tuple)) else [ageom]
         for a, b in zip(ageom, self.ageom):
             if a.ns != b.ns or a.nr != b.nr:

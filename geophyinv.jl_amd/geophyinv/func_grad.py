@@ -47,6 +47,17 @@ def gradient(g, m, loss, dobs, pa, mparams=None) -> float:
     pa.update_model(m, mparams)
     mode_save = _forward_save(pa)
     J = database.lossvalue(loss, dobs, pa.data)
+    backprop(g, m, loss, dobs, pa, mparams)
+    pa.attrib_mod.mode = mode_save
+    return database.lossvalue(loss, dobs, pa.data) if pa.compat == 1 else J
+
+
+def backprop(g, m, loss, dobs, pa, mparams=None) -> None:
+    """The adjoint half of ``gradient!`` (func_grad.jl:21-45): from the records and stored
+    boundary values of the :forward_save run just made at ``m``, back-propagate the adjoint
+    sources and write the chain-ruled gradient into ``g``.  Leaves ``pa`` in the :adjoint
+    mode with its data zeroed, as the reference's ``update!`` does."""
+    mparams = list(mparams or pa.mparams)
     # adjoint sources, time reversed
     database.gradient(pa.srcwav2, loss, dobs, pa.data)
     for S in pa.srcwav2:
@@ -67,5 +78,3 @@ def gradient(g, m, loss, dobs, pa, mparams=None) -> float:
         gm32 = (gm.astype(np.float64) * np.exp(xr) * np.float64(pa.ref_mod[name])).astype(F32)
         out.append(gm32.ravel(order="F"))
     g[...] = np.concatenate(out)
-    pa.attrib_mod.mode = mode_save
-    return database.lossvalue(loss, dobs, pa.data) if pa.compat == 1 else J

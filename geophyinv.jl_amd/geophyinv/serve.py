@@ -22,11 +22,13 @@ F32 = np.float32
 
 class Server:
     def __init__(self, physics: int = 0, order: int = 2, *, max_shots: int = 64, chunk_steps: int = 32,
-                 autotune: bool = True, tile: int = 0, hbm_budget_bytes: int = 0, device: int = 0):
+                 autotune: bool = True, tile: int = 0, hbm_budget_bytes: int = 0, device: int = 0,
+                 graphs: bool = True):
         self.lib = _lib.load()
         o = _lib.gpi_server_opts()
         o.physics, o.order, o.max_shots, o.chunk_steps = physics, order, max_shots, chunk_steps
         o.autotune, o.tile, o.hbm_budget_bytes = int(bool(autotune)), tile, hbm_budget_bytes
+        o.direct_launch = 0 if graphs else 1
         h = C.c_void_p()
         rc = self.lib.gpi_server_create(device, C.byref(o), C.byref(h))
         if rc != _lib.GPI_OK:

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define GPI_ABI_VERSION 3
+#define GPI_ABI_VERSION 4
 
 /* error codes */
 #define GPI_OK 0
@@ -292,9 +292,13 @@ typedef struct gpi_server_opts {
     int32_t physics;             /* 0 acoustic, 1 elastic */
     int32_t order;               /* 2, 4, 6, 8 */
     int32_t max_shots;           /* resident shots at most (default 64) */
-    int32_t chunk_steps;         /* time steps between admissions (default 32) */
+    int32_t chunk_steps;         /* time steps between admissions (default 32, at most 4096) */
     int32_t autotune;            /* 1: pick the workgroup shape per batch size by timing */
     int32_t tile;                /* fixed shape index when autotune = 0 (0..3) */
+    int32_t direct_launch;       /* 0: a chunk whose batch and shape are those of the chunk
+                                    before it replays a captured HIP graph of its 2 x
+                                    chunk_steps launches; 1: every pass launched directly */
+    int32_t reserved;            /* 0 */
     uint64_t hbm_budget_bytes;   /* 0: 90 % of the free HBM at creation */
 } gpi_server_opts;
 
@@ -320,6 +324,8 @@ typedef struct gpi_server_stats {
     double mean_batch;           /* mean resident shots per step */
     int32_t tile_ty, tile_rx;    /* workgroup shape of the last chunk */
     uint64_t hbm_used, hbm_budget;
+    int64_t graph_chunks;        /* chunks run as a graph replay */
+    int64_t graph_captures;      /* graphs captured and instantiated */
 } gpi_server_stats;
 
 int gpi_server_create(int device, const gpi_server_opts* opts, gpi_server** out);

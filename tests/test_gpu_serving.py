@@ -89,10 +89,18 @@ def run_server(ctx, shots, **kw):
 def test_server_alone_equals_mix_fixed_tiles(gpu_ctx, tile):
     shots = [shot(gpu_ctx, 100 + k) for k in range(6)]
     st, worst = run_server(gpu_ctx, shots, autotune=False, tile=tile, chunk_steps=16)
-    print(f"tile {tile}: {st['completed']} shots, mean batch {st['mean_batch']:.2f}, worst rel L2 {worst:.1e}",
-          flush=True)
+    print(f"tile {tile}: {st['completed']} shots, mean batch {st['mean_batch']:.2f}, worst rel L2 {worst:.1e}, "
+          f"{st['graph_chunks']} graph chunks / {st['graph_captures']} captures", flush=True)
     assert st["completed"] == 6 and st["failed"] == 0
     assert st["mean_batch"] > 1.5    # shots did overlap on the device
+    # chunks whose batch did not change replayed captured graphs, some graphs more than once
+    assert st["graph_captures"] >= 1 and st["graph_chunks"] > st["graph_captures"]
+
+
+def test_server_direct_launch_equals_plain(gpu_ctx):
+    shots = [shot(gpu_ctx, 100 + k) for k in range(6)]
+    st, _ = run_server(gpu_ctx, shots, autotune=False, tile=0, chunk_steps=16, graphs=False)
+    assert st["completed"] == 6 and st["graph_chunks"] == 0 and st["graph_captures"] == 0
 
 
 def test_server_autotune_orders_and_staggered_admission(gpu_ctx):
