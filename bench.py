@@ -89,6 +89,12 @@ def cpu_baseline(pa, seconds):
                       f"first {steps} of {len(pa.tgrid)} time steps, {dt:.1f} s"}
 
 
+def _digest(recs):
+    """Checksums of one shot's records (compared between the plain and served runs)."""
+    import xxhash
+    return tuple(xxhash.xxh3_64_intdigest(memoryview(recs[f].reshape(-1, order="A")).cast("B")) for f in sorted(recs))
+
+
 def _experiment(G, rq, **kw):
     return G.SeisForwExpt(G.FdtdAcoustic(), medium=rq["medium"], tgrid=rq["tgrid"], ageom=rq["ageom"],
                           srcwav=rq["srcwav"], **kw, **rq["kw"])
@@ -103,7 +109,7 @@ def run_plain(reqs, ctx, G, prebuilt=False):
 
     pas = [_experiment(G, rq, context=ctx, coefficients="host") for rq in reqs] if prebuilt else None
     t0 = time.perf_counter()
-    lat, sums = [], []
+    lat, out = [], []
     for k, rq in enumerate(reqs):
         t_arr = t0 + rq["arrival"]
         now = time.perf_counter()
@@ -113,9 +119,9 @@ def run_plain(reqs, ctx, G, prebuilt=False):
         recs, _ = pa.run_shot(0)
         pa.free()
         lat.append(time.perf_counter() - t_arr)
-        sums.append(tuple(hash(recs[f].tobytes()) for f in sorted(recs)))
+        out.append(recs)
     span = time.perf_counter() - t0
-    return span, np.array(lat), sums
+    return span, np.array(lat), [_digest(r) for r in out]
 
 
 def run_served(reqs, G, opts, prebuilt=False):
@@ -140,14 +146,13 @@ def run_served(reqs, G, opts, prebuilt=False):
             lag = time.perf_counter() - t_arr
             tickets.append((srv.submit(pa, 0), lag))
             del pa
-        lat, sums = [], []
+        lat, out = [], []
         for t, lag in tickets:
-            recs = srv.wait(t)
+            out.append(srv.wait(t))
             lat.append(lag + srv.latency_ms(t) * 1e-3)
-            sums.append(tuple(hash(recs[f].tobytes()) for f in sorted(recs)))
         span = time.perf_counter() - t0
         st = srv.stats()
-    return span, np.array(lat), sums, st
+    return span, np.array(lat), [_digest(r) for r in out], st
 
 
 def serving_section(nshots, opts, G, _lib, side=(256, 2048), steps=(1000, 6000), compare_direct=False,
@@ -188,7 +193,8 @@ def serving_section(nshots, opts, G, _lib, side=(256, 2048), steps=(1000, 6000),
                    "gpt_per_s": round(cells / s_span / 1e9, 2), "device_busy_s": round(st["busy_ms"] * 1e-3, 2),
                    "mean_batch": round(st["mean_batch"], 2), "tile": [st["tile_ty"], st["tile_rx"]],
                    "graph_chunks": st["graph_chunks"], "graph_captures": st["graph_captures"],
-                   "batched_steps": st["steps"], "options": opts},
+                   "batched_steps": st["steps"], "options": opts,
+                   "host_ms": {k[5:-3]: round(st[k], 1) for k in st if k.startswith("host_")}},
         "served_direct_launch": direct,
         "speedup_shots_per_s": round(p_span / s_span, 3),
         "records_identical": f"{same}/{nshots}",

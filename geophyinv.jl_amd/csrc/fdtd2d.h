@@ -66,10 +66,12 @@ struct BatchShot2D {
     int64_t nt;
 };
 
-// One pass of every shot of a device table: first[s] = first workgroup of shot s,
-// first[n] = nblocks; all shots share physics, order and tile shape.
+// One pass of every shot of a device table: n = *nshots shots, first[s] = first workgroup
+// of shot s, first[n] = the workgroups they need; the launch has nblocks >= first[n]
+// workgroups.  All shots share physics, order and tile shape.
 void batch_pass_2d(int physics, int order, const TileShape& t, bool velocity, const BatchShot2D* D,
-                   const int* first, int n, int nblocks, const int64_t* step0, int koff, hipStream_t s);
+                   const int* first, const int* nshots, int nblocks, const int64_t* step0, int koff,
+                   hipStream_t s);
 // *step0 = step, stream-ordered (a one-thread kernel: no memset node in a captured chunk).
 void batch_set_step(int64_t* step0, int64_t step, hipStream_t s);
 

@@ -421,7 +421,9 @@ void launch_str(const Model& m, hipStream_t s, StrParams2D P, const PassExtras& 
 // idle, and at it == nt a shot takes the trailing velocity half step of
 // src/fdtd/propagate.jl:254-255 (no sources, no receivers).  The batch's step is read from
 // device memory (*step0, set by batch_set_step before a chunk) plus the launch's offset
-// koff in the chunk, so one captured graph of a chunk's launches replays at any step.
+// koff in the chunk, and so is the number of shots (*nshots); a launch may cover more
+// workgroups than first[n] (the extra ones return at once).  One captured graph of a
+// chunk's launches therefore replays at any step and for any batch whose workgroups fit.
 // ---------------------------------------------------------------------------------------
 
 __device__ __forceinline__ int shot_of_block(const int* __restrict__ first, int n, int b)
@@ -437,10 +439,13 @@ __device__ __forceinline__ int shot_of_block(const int* __restrict__ first, int 
 
 template <int O, int PHYS, int TY, int RX>
 __global__ __launch_bounds__(64 * TY) void vel2d_batch(const BatchShot2D* __restrict__ D, const int* __restrict__ first,
-                                                       int n, const int64_t* __restrict__ step0, int koff)
+                                                       const int* __restrict__ nshots,
+                                                       const int64_t* __restrict__ step0, int koff)
 {
     const int64_t step = *step0 + koff;
+    const int n = *nshots;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
+    if (b >= first[n]) return;
     const int s = shot_of_block(first, n, b);
     const BatchShot2D& S = D[s];
     const int64_t it = step - S.start;
@@ -456,10 +461,13 @@ __global__ __launch_bounds__(64 * TY) void vel2d_batch(const BatchShot2D* __rest
 
 template <int O, int PHYS, int TY, int RX>
 __global__ __launch_bounds__(64 * TY) void str2d_batch(const BatchShot2D* __restrict__ D, const int* __restrict__ first,
-                                                       int n, const int64_t* __restrict__ step0, int koff)
+                                                       const int* __restrict__ nshots,
+                                                       const int64_t* __restrict__ step0, int koff)
 {
     const int64_t step = *step0 + koff;
+    const int n = *nshots;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
+    if (b >= first[n]) return;
     const int s = shot_of_block(first, n, b);
     const BatchShot2D& S = D[s];
     const int64_t it = step - S.start;
@@ -479,7 +487,7 @@ __global__ void batch_set_step_kernel(int64_t* step0, int64_t step)
 }
 
 template <int O, int PHYS, int TY, int RX>
-void launch_batch_t(bool velocity, const BatchShot2D* D, const int* first, int n, int nblocks,
+void launch_batch_t(bool velocity, const BatchShot2D* D, const int* first, const int* n, int nblocks,
                     const int64_t* step0, int koff, hipStream_t s)
 {
     dim3 grd(nblocks), blk(64, TY);
@@ -488,8 +496,8 @@ void launch_batch_t(bool velocity, const BatchShot2D* D, const int* first, int n
 }
 
 template <int O, int PHYS>
-void launch_batch_o(const TileShape& t, bool velocity, const BatchShot2D* D, const int* first, int n, int nblocks,
-                    const int64_t* step0, int koff, hipStream_t s)
+void launch_batch_o(const TileShape& t, bool velocity, const BatchShot2D* D, const int* first, const int* n,
+                    int nblocks, const int64_t* step0, int koff, hipStream_t s)
 {
     if (t.ty == 4 && t.rx == 1) launch_batch_t<O, PHYS, 4, 1>(velocity, D, first, n, nblocks, step0, koff, s);
     else if (t.ty == 8 && t.rx == 1) launch_batch_t<O, PHYS, 8, 1>(velocity, D, first, n, nblocks, step0, koff, s);
@@ -526,7 +534,7 @@ void stress_grid_2d(const Model& m, const TileShape& t, int* gx, int* gy)
 }
 
 void batch_pass_2d(int physics, int order, const TileShape& t, bool velocity, const BatchShot2D* D, const int* first,
-                   int n, int nblocks, const int64_t* step0, int koff, hipStream_t s)
+                   const int* n, int nblocks, const int64_t* step0, int koff, hipStream_t s)
 {
     if (nblocks <= 0) return;
     if (physics == 0) {

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -85,9 +87,42 @@ struct Store {
     uint64_t bytes = 0;
 };
 
+// Sub-allocator of device memory for contexts that allocate and free at a high rate (the
+// serving scheduler): segments from hipMalloc, carved first-fit into 256-byte aligned
+// blocks; a freed block coalesces with its free neighbours.  Not thread-safe: one thread
+// (the scheduler) allocates and frees.
+class DevicePool {
+public:
+    ~DevicePool();
+    hipError_t alloc(void** p, size_t bytes);
+    void free(void* p);
+    void trim();                           // hipFree every segment with no live block
+    uint64_t reserved() const { return reserved_; }
+
+private:
+    struct Seg {
+        char* base;
+        size_t size, live;
+        std::map<size_t, size_t> free;     // offset -> size
+    };
+    std::vector<Seg*> segs_;
+    std::unordered_map<void*, std::pair<Seg*, size_t>> blocks_;   // block -> (segment, size)
+    uint64_t reserved_ = 0;
+    bool carve(Seg* g, size_t n, void** p);
+};
+
+// Pinned staging of host-to-device copies (copy_h2d): the source is copied into the ring
+// and the upload is queued on the context's stream; the ring wraps after the stream drained.
+struct PinnedStage {
+    char* buf = nullptr;
+    size_t cap = 0, off = 0;
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    DevicePool* pool = nullptr;          // when set, dev_alloc carves from it
+    PinnedStage* stage = nullptr;        // when set, copy_h2d is asynchronous on `stream`
     // When set, frees are deferred: pointers and bytes go here and the owner releases them
     // once no queued work can reference them (the serving scheduler drains its streams first).
     std::vector<void*>* defer_free = nullptr;

@@ -10,10 +10,11 @@
 // only after both streams have drained, so no queued launch can reference freed memory.
 // At most two chunks are in flight, so admission and copy-back overlap the device work.
 // The workgroup shape of the batched kernels is chosen per batch size by timing each
-// candidate on live chunks.  A full chunk whose batch and shape are those of the chunk
-// before it is one hipGraphLaunch of a captured graph of its 2 x chunk_steps launches
-// (cached per batch layout, least recently used evicted); the kernels read the batch step
-// from device memory, set before each chunk, so a graph replays at any step.
+// candidate on live chunks.  A full chunk is one hipGraphLaunch of a captured graph of its
+// 2 x chunk_steps launches: the kernels read the batch step and shot count from device
+// memory and the launches cover a bucketed workgroup count (extra workgroups return), so
+// one graph per shape and bucket serves every batch composition (least recently used
+// graphs evicted).
 //
 // No reference counterpart: the reference propagates supersources one after another on
 // each worker (src/fdtd/propagate.jl:100-106, :138-261).
@@ -73,10 +74,12 @@ struct Slot {
 };
 
 struct PinnedPool {
+    std::mutex mu;   // get() on the scheduler thread, put() on the completion thread
     std::vector<std::pair<float*, size_t>> free_list;
     std::vector<float*> all;
     float* get(size_t count, size_t* cap)
     {
+        std::lock_guard<std::mutex> lk(mu);
         for (size_t k = 0; k < free_list.size(); ++k)
             if (free_list[k].second >= count) {
                 float* p = free_list[k].first;
@@ -91,7 +94,11 @@ struct PinnedPool {
         *cap = n;
         return (float*)p;
     }
-    void put(float* p, size_t cap) { free_list.push_back({p, cap}); }
+    void put(float* p, size_t cap)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        free_list.push_back({p, cap});
+    }
     ~PinnedPool()
     {
         for (float* p : all) (void)hipHostFree(p);
@@ -123,6 +130,14 @@ struct gpi_server {
     std::vector<void*> grave;     // device memory of departed shots, freed once drained
     uint64_t grave_bytes = 0;
     std::thread th;
+    // completion thread: hands finished shots' records to the caller off the scheduler
+    std::thread fin_th;
+    std::mutex fin_mu;
+    std::condition_variable fin_cv;
+    std::deque<std::pair<Slot*, int>> fin_q;
+    bool fin_stop = false;
+    void complete(Slot* s, int code);   // queue a finished (or failed) shot for finalize()
+    void run_completions();
     std::mutex mu;
     std::condition_variable cv, cvd;
     bool stop = false;
@@ -134,6 +149,17 @@ struct gpi_server {
     // statistics (guarded by mu)
     int64_t submitted = 0, completed = 0, failed = 0, steps = 0, launches = 0;
     int64_t graph_chunks = 0, graph_captures = 0;
+    double t_admit = 0.0, t_launch = 0.0, t_retire = 0.0, t_finalize = 0.0, t_collect = 0.0;   // host ms
+    // the scheduler's own accumulators, published into t_* under mu once per chunk
+    double h_admit = 0.0, h_launch = 0.0, h_retire = 0.0, h_collect = 0.0;
+    void publish_times()
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        t_admit = h_admit;
+        t_launch = h_launch;
+        t_retire = h_retire;
+        t_collect = h_collect;
+    }
     double busy_ms = 0.0, batch_steps = 0.0;
     std::vector<double> latency;
     std::map<uint64_t, double> lat_of;   // per finished ticket
@@ -153,63 +179,127 @@ struct gpi_server {
     std::map<int, int> best_shape; // batch-size class -> shape index
     // captured chunks: one per batch layout (shape, shots, workgroups of both passes)
     struct ChunkGraph {
-        int shape, ns, nbv, nbs;
+        int shape, nbv, nbs;   // workgroup buckets of the two passes
         hipGraphExec_t exec;
         uint64_t used;
     };
-    static constexpr size_t kMaxGraphs = 8;
+    static constexpr size_t kMaxGraphs = 16;
     std::vector<ChunkGraph> graphs;
     std::vector<hipGraphExec_t> graph_grave;   // evicted; destroyed once the streams drain
     uint64_t graph_clock = 0;
     hipStream_t cap = nullptr;                 // capture stream (never runs work)
     int64_t* dstep = nullptr;                  // the batch step the kernels read
+    // device memory of both contexts comes from `pool`; freed blocks return to it once an
+    // event recorded after their last use on the stream that used them has completed
+    DevicePool pool_dev;
+    PinnedStage stage;
+    struct Pending {
+        hipEvent_t ev;
+        std::vector<void*> ptrs;
+        uint64_t bytes;
+    };
+    std::deque<Pending> pending;
+    uint64_t pending_bytes = 0;
+    void defer_grave(hipStream_t st);
+    void release_done();
 
     void run();
-    void collect();
+    void collect(bool idle);
     int admit(std::unique_ptr<Req> r, Slot** out);
     void retire(Slot* s);
     void finalize(Slot* s, int code);
     void fail_req(uint64_t ticket, int code, const std::string& msg);
     int upload_table(const TileShape& t, int* nbv, int* nbs);
-    int chunk_graph(int shape, int ns, int nbv, int nbs, hipGraphExec_t* out);
-    void launch_chunk(const TileShape& t, int ns, int nbv, int nbs, int64_t K, hipStream_t st);
+    int chunk_graph(int shape, int bv, int bs, hipGraphExec_t* out);
+    void launch_chunk(const TileShape& t, int nbv, int nbs, int64_t K, hipStream_t st);
 };
 
-void gpi_server::collect()
+// Blocks freed since the last call were last used by work queued on `st`: they return to
+// the pool once an event recorded now has completed.
+void gpi_server::defer_grave(hipStream_t st)
 {
-    if (grave.empty() && graph_grave.empty()) return;
+    if (grave.empty()) return;
+    Pending e{nullptr, {}, grave_bytes - pending_bytes};
+    if (hipEventCreateWithFlags(&e.ev, hipEventDisableTiming) != hipSuccess || hipEventRecord(e.ev, st) != hipSuccess) {
+        if (e.ev) (void)hipEventDestroy(e.ev);
+        return;   // left in the graveyard for collect()
+    }
+    e.ptrs.swap(grave);
+    pending_bytes += e.bytes;
+    pending.push_back(std::move(e));
+}
+
+void gpi_server::release_done()
+{
+    for (auto it = pending.begin(); it != pending.end();) {
+        if (hipEventQuery(it->ev) != hipSuccess) {
+            ++it;
+            continue;
+        }
+        for (void* p : it->ptrs) pool_dev.free(p);
+        (void)hipEventDestroy(it->ev);
+        grave_bytes -= std::min(grave_bytes, it->bytes);
+        pending_bytes -= std::min(pending_bytes, it->bytes);
+        it = pending.erase(it);
+    }
+}
+
+// Drain both streams and return every freed block to the pool; when the server is idle
+// also give empty pool segments back to the device.
+void gpi_server::collect(bool idle)
+{
+    if (grave.empty() && pending.empty() && graph_grave.empty() && !(idle && pool_dev.reserved())) return;
+    const double t0 = now_ms();
     (void)hipStreamSynchronize(ac->stream);
     (void)hipStreamSynchronize(cc->stream);
-    for (void* p : grave) (void)hipFree(p);
+    for (Pending& e : pending) {
+        for (void* p : e.ptrs) pool_dev.free(p);
+        (void)hipEventDestroy(e.ev);
+    }
+    pending.clear();
+    pending_bytes = 0;
+    for (void* p : grave) pool_dev.free(p);
     grave.clear();
     grave_bytes = 0;
     for (hipGraphExec_t g : graph_grave) (void)hipGraphExecDestroy(g);
     graph_grave.clear();
+    if (idle) pool_dev.trim();
+    h_collect += now_ms() - t0;
 }
 
-// The 2 x chunk_steps launches of one full chunk, kernel offsets 0 .. chunk_steps - 1.
-void gpi_server::launch_chunk(const TileShape& t, int ns, int nbv, int nbs, int64_t K, hipStream_t st)
+// The 2 x K launches of a chunk (kernel offsets 0 .. K - 1) over nbv and nbs workgroups.
+void gpi_server::launch_chunk(const TileShape& t, int nbv, int nbs, int64_t K, hipStream_t st)
 {
+    const int* nsh = dfirst + 2 * (o.max_shots + 1);
     for (int64_t k = 0; k < K; ++k) {
-        batch_pass_2d(o.physics, o.order, t, true, dtab, dfirst, ns, nbv, dstep, (int)k, st);
-        batch_pass_2d(o.physics, o.order, t, false, dtab, dfirst + o.max_shots + 1, ns, nbs, dstep, (int)k, st);
+        batch_pass_2d(o.physics, o.order, t, true, dtab, dfirst, nsh, nbv, dstep, (int)k, st);
+        batch_pass_2d(o.physics, o.order, t, false, dtab, dfirst + o.max_shots + 1, nsh, nbs, dstep, (int)k, st);
     }
 }
 
-// The executable graph of a full chunk for this batch layout, captured on first use.  The
-// table contents may change between replays (they are copied before the chunk that needs
-// them); the table and step addresses the launches read are fixed for the server's life.
-int gpi_server::chunk_graph(int shape, int ns, int nbv, int nbs, hipGraphExec_t* out)
+// Workgroup count of a captured launch: the smallest of 256, 320, 384, ... (steps of about
+// 1/8, multiples of 64) that holds nb, so batches of similar size share one graph.
+int launch_bucket(int nb)
+{
+    int v = 256;
+    while (v < nb) v = (v + v / 8 + 63) / 64 * 64;
+    return v;
+}
+
+// The executable graph of a full chunk for this shape and workgroup buckets, captured on
+// first use.  Everything its launches read (table, shot count, step) lives at addresses
+// fixed for the server's life and is written before the chunk that needs it.
+int gpi_server::chunk_graph(int shape, int bv, int bs, hipGraphExec_t* out)
 {
     for (ChunkGraph& g : graphs)
-        if (g.shape == shape && g.ns == ns && g.nbv == nbv && g.nbs == nbs) {
+        if (g.shape == shape && g.nbv == bv && g.nbs == bs) {
             g.used = ++graph_clock;
             *out = g.exec;
             return GPI_OK;
         }
     hipGraph_t gr = nullptr;
     HIPTRY(cc, hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed));
-    launch_chunk(TILE_SHAPES[shape], ns, nbv, nbs, o.chunk_steps, cap);
+    launch_chunk(TILE_SHAPES[shape], bv, bs, o.chunk_steps, cap);
     const hipError_t le = hipGetLastError();
     const hipError_t ee = hipStreamEndCapture(cap, &gr);
     if (le != hipSuccess || ee != hipSuccess) {
@@ -226,7 +316,7 @@ int gpi_server::chunk_graph(int shape, int ns, int nbv, int nbs, hipGraphExec_t*
         graph_grave.push_back(lru->exec);   // may still be queued on the compute stream
         graphs.erase(lru);
     }
-    graphs.push_back(ChunkGraph{shape, ns, nbv, nbs, ex, ++graph_clock});
+    graphs.push_back(ChunkGraph{shape, bv, bs, ex, ++graph_clock});
     {
         std::lock_guard<std::mutex> lk(mu);
         ++graph_captures;
@@ -304,20 +394,54 @@ void gpi_server::retire(Slot* s)
     s->S = nullptr;
     free_model(cc, s->m);
     s->m = nullptr;
+    defer_grave(cc->stream);
+}
+
+void gpi_server::complete(Slot* s, int code)
+{
+    {
+        std::lock_guard<std::mutex> lk(fin_mu);
+        fin_q.emplace_back(s, code);
+    }
+    fin_cv.notify_one();
+}
+
+void gpi_server::run_completions()
+{
+    for (;;) {
+        std::pair<Slot*, int> job;
+        {
+            std::unique_lock<std::mutex> lk(fin_mu);
+            fin_cv.wait(lk, [&] { return fin_stop || !fin_q.empty(); });
+            if (fin_q.empty()) return;   // stopping and drained
+            job = fin_q.front();
+            fin_q.pop_front();
+        }
+        finalize(job.first, job.second);
+    }
 }
 
 void gpi_server::finalize(Slot* s, int code)
 {
     const Req& q = *s->r;
-    if (code == GPI_OK) {
+    if (code == GPI_OK) {   // staging holds [it][r] per field, the caller [r][it]: 32 x 32 blocks
+        const double t0 = now_ms();
         size_t off = 0;
         for (size_t k = 0; k < s->rec.size(); ++k) {
             const int np = s->rec[k].npts;
             float* o = q.out[k];
-            for (int r = 0; r < np; ++r)
-                for (int64_t it = 0; it < q.nt; ++it) o[(size_t)r * q.nt + it] = s->pin[off + (size_t)it * np + r];
+            const float* in = s->pin + off;
+            for (int64_t i0 = 0; i0 < q.nt; i0 += 32)
+                for (int r0 = 0; r0 < np; r0 += 32) {
+                    const int64_t i1 = std::min<int64_t>(i0 + 32, q.nt);
+                    const int r1 = std::min(r0 + 32, np);
+                    for (int r = r0; r < r1; ++r)
+                        for (int64_t it = i0; it < i1; ++it) o[(size_t)r * q.nt + it] = in[(size_t)it * np + r];
+                }
             off += (size_t)q.nt * (size_t)std::max(np, 1);
         }
+        std::lock_guard<std::mutex> lk(mu);
+        t_finalize += now_ms() - t0;
     }
     pool.put(s->pin, s->pin_cap);
     (void)hipEventDestroy(s->ev);
@@ -368,8 +492,10 @@ int gpi_server::upload_table(const TileShape& t, int* nbv, int* nbs)
     }
     fv[n] = nv;
     fs[n] = ns;
+    fv[2 * (o.max_shots + 1)] = n;   // the shot count the kernels read
     HIPTRY(cc, hipMemcpyAsync(dtab, T, sizeof(BatchShot2D) * n, hipMemcpyHostToDevice, cc->stream));
-    HIPTRY(cc, hipMemcpyAsync(dfirst, fv, sizeof(int) * 2 * (o.max_shots + 1), hipMemcpyHostToDevice, cc->stream));
+    HIPTRY(cc, hipMemcpyAsync(dfirst, fv, sizeof(int) * (2 * (o.max_shots + 1) + 1), hipMemcpyHostToDevice,
+                              cc->stream));
     HIPTRY(cc, hipEventRecord(tab_ev[tab_cur], cc->stream));
     tab_cur ^= 1;
     *nbv = nv;
@@ -403,7 +529,7 @@ void gpi_server::run()
         for (Slot* s : active) {
             retire(s);
             (void)hipStreamSynchronize(cc->stream);
-            finalize(s, code);
+            complete(s, code);
         }
         active.clear();
         std::lock_guard<std::mutex> lk(mu);
@@ -415,7 +541,8 @@ void gpi_server::run()
         // ---- admission ----------------------------------------------------------------
         std::vector<std::unique_ptr<Req>> take;
         bool blocked = false;
-        if (active.empty() && finishing.empty()) collect();   // idle: release departed shots
+        release_done();
+        if (active.empty() && finishing.empty()) collect(true);   // idle: release departed shots
         {
             std::unique_lock<std::mutex> lk(mu);
             if (active.empty() && finishing.empty() && queue.empty()) {
@@ -435,13 +562,17 @@ void gpi_server::run()
             }
         }
         // memory of departed shots holds back admission: drain and release it
-        if ((blocked && grave_bytes > 0) || grave_bytes > budget / 8 || graph_grave.size() >= kMaxGraphs) collect();
+        if ((blocked && grave_bytes > 0) || grave_bytes > budget / 8 || graph_grave.size() >= kMaxGraphs)
+            collect(false);
         bool admitted = false;
         for (auto& r : take) {
             const uint64_t ticket = r->ticket;
             const uint64_t est = r->est;
             Slot* s = nullptr;
+            const double t0 = now_ms();
             const int rc = admit(std::move(r), &s);
+            defer_grave(ac->stream);
+            h_admit += now_ms() - t0;
             if (rc) {
                 {
                     std::lock_guard<std::mutex> lk(mu);
@@ -467,7 +598,7 @@ void gpi_server::run()
         for (size_t k = 0; k < finishing.size();) {
             Slot* s = finishing[k];
             if (hipEventQuery(s->ev) == hipSuccess) {
-                finalize(s, GPI_OK);
+                complete(s, GPI_OK);
                 finishing.erase(finishing.begin() + k);
             } else {
                 ++k;
@@ -497,7 +628,7 @@ void gpi_server::run()
             }
         }
         const TileShape t = TILE_SHAPES[shape];
-        const bool fresh = dirty;   // a new batch layout or shape: launched directly once
+        const double tl0 = now_ms();
         if (dirty) {
             if (upload_table(t, &nbv, &nbs)) {
                 shot_fail_all(GPI_EHIP, cc->err);
@@ -519,9 +650,9 @@ void gpi_server::run()
         const int ns = (int)active.size();
         batch_set_step(dstep, step, cc->stream);
         bool graphed = false;
-        if (!o.direct_launch && !fresh && K == o.chunk_steps) {
+        if (!o.direct_launch && K == o.chunk_steps) {
             hipGraphExec_t ex = nullptr;
-            if (chunk_graph(shape, ns, nbv, nbs, &ex)) {
+            if (chunk_graph(shape, launch_bucket(nbv), launch_bucket(nbs), &ex)) {
                 shot_fail_all(GPI_EHIP, cc->err);
                 continue;
             }
@@ -532,7 +663,7 @@ void gpi_server::run()
             }
             graphed = true;
         } else {
-            launch_chunk(t, ns, nbv, nbs, K, cc->stream);
+            launch_chunk(t, nbv, nbs, K, cc->stream);
         }
         (void)hipEventRecord(E[1], cc->stream);
         const hipError_t le = hipGetLastError();
@@ -542,11 +673,16 @@ void gpi_server::run()
         }
         step += K;
         ++chunk_no;
+        h_launch += now_ms() - tl0;
         {
             std::lock_guard<std::mutex> lk(mu);
             steps += K;
             launches += 2 * K;
             graph_chunks += graphed ? 1 : 0;
+            t_admit = h_admit;
+            t_launch = h_launch;
+            t_retire = h_retire;
+            t_collect = h_collect;
             batch_steps += (double)ns * (double)K;
             last_ty = t.ty;
             last_rx = t.rx;
@@ -573,7 +709,9 @@ void gpi_server::run()
         for (size_t k = 0; k < active.size();) {
             Slot* s = active[k];
             if (step - s->start > s->r->nt) {
+                const double t0 = now_ms();
                 retire(s);
+                h_retire += now_ms() - t0;
                 finishing.push_back(s);
                 active.erase(active.begin() + k);
                 dirty = true;
@@ -585,7 +723,8 @@ void gpi_server::run()
     (void)hipStreamSynchronize(cc->stream);
     for (int k = 0; k < 2; ++k)
         if (pend[k] > 0) account(ck_ev[k][0], ck_ev[k][1], pend[k]);
-    collect();
+    collect(true);
+    publish_times();
 }
 
 extern "C" {
@@ -610,6 +749,8 @@ int gpi_server_create(int device, const gpi_server_opts* opts, gpi_server** out)
         delete v;
         return rc;
     }
+    v->cc->pool = v->ac->pool = &v->pool_dev;
+    v->ac->stage = &v->stage;
     v->cc->defer_free = &v->grave;
     v->cc->defer_bytes = &v->grave_bytes;
     v->ac->defer_free = &v->grave;
@@ -619,9 +760,9 @@ int gpi_server_create(int device, const gpi_server_opts* opts, gpi_server** out)
     size_t fr = 0, tot = 0;
     (void)hipMemGetInfo(&fr, &tot);
     v->budget = o.hbm_budget_bytes ? o.hbm_budget_bytes : (uint64_t)((double)fr * 0.9);
-    const size_t tb = sizeof(BatchShot2D) * v->o.max_shots + sizeof(int) * 2 * (v->o.max_shots + 1);
+    const size_t tb = sizeof(BatchShot2D) * v->o.max_shots + sizeof(int) * (2 * (v->o.max_shots + 1) + 1);
     bool ok = hipMalloc((void**)&v->dtab, sizeof(BatchShot2D) * v->o.max_shots) == hipSuccess &&
-              hipMalloc((void**)&v->dfirst, sizeof(int) * 2 * (v->o.max_shots + 1)) == hipSuccess &&
+              hipMalloc((void**)&v->dfirst, sizeof(int) * (2 * (v->o.max_shots + 1) + 1)) == hipSuccess &&
               hipMalloc((void**)&v->dstep, sizeof(int64_t)) == hipSuccess &&
               hipStreamCreateWithFlags(&v->cap, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreateWithFlags(&v->adm_ev, hipEventDisableTiming) == hipSuccess;
@@ -635,6 +776,7 @@ int gpi_server_create(int device, const gpi_server_opts* opts, gpi_server** out)
         gpi_server_destroy(v);
         return GPI_EHIP;
     }
+    v->fin_th = std::thread([v] { v->run_completions(); });
     v->th = std::thread([v] { v->run(); });
     *out = v;
     return GPI_OK;
@@ -650,6 +792,14 @@ void gpi_server_destroy(gpi_server* v)
         }
         v->cv.notify_all();
         v->th.join();
+    }
+    if (v->fin_th.joinable()) {   // the scheduler has handed over every shot: drain, then stop
+        {
+            std::lock_guard<std::mutex> lk(v->fin_mu);
+            v->fin_stop = true;
+        }
+        v->fin_cv.notify_all();
+        v->fin_th.join();
     }
     if (v->cc) {
         (void)hipSetDevice(v->cc->device);
@@ -670,7 +820,8 @@ void gpi_server_destroy(gpi_server* v)
     if (v->adm_ev) (void)hipEventDestroy(v->adm_ev);
     if (v->ac) gpi_destroy(v->ac);
     if (v->cc) gpi_destroy(v->cc);
-    delete v;
+    if (v->stage.buf) (void)hipHostFree(v->stage.buf);   // its copies ran on the admission stream
+    delete v;                                             // the pool's segments go with it
 }
 
 const char* gpi_server_error(const gpi_server* v) { return v ? v->err.c_str() : "null server"; }
@@ -817,6 +968,12 @@ int gpi_server_get_stats(gpi_server* v, gpi_server_stats* out)
     s.launches = v->launches;
     s.graph_chunks = v->graph_chunks;
     s.graph_captures = v->graph_captures;
+    s.host_admit_ms = v->t_admit;
+    s.host_launch_ms = v->t_launch;
+    s.host_retire_ms = v->t_retire;
+    s.host_finalize_ms = v->t_finalize;
+    s.host_collect_ms = v->t_collect;
+    s.pool_bytes = v->pool_dev.reserved();
     s.busy_ms = v->busy_ms;
     s.mean_batch = v->steps ? v->batch_steps / (double)v->steps : 0.0;
     s.tile_ty = v->last_ty;

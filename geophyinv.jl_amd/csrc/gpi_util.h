@@ -103,7 +103,8 @@ int create_model_medium(gpi_ctx* c, const gpi_grid* g, const gpi_medium* med, co
 int prep_coefficients_2d(gpi_ctx* c, Model& m, const float* vp, const float* vs, const float* rho,
                          const int64_t nm[3], hipStream_t s);
 
-// blocking host -> device copy
+// host -> device copy: blocking, or queued on c->stream through the context's pinned
+// staging ring when it has one (the source may be reused as soon as this returns)
 int copy_h2d(gpi_ctx* c, void* dst, const void* src, size_t bytes);
 
 int upload_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3]);

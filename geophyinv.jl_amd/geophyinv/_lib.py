@@ -85,7 +85,9 @@ class gpi_server_stats(C.Structure):
                 ("busy_ms", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
                 ("mean_batch", C.c_double), ("tile_ty", C.c_int32), ("tile_rx", C.c_int32),
                 ("hbm_used", C.c_uint64), ("hbm_budget", C.c_uint64), ("graph_chunks", C.c_int64),
-                ("graph_captures", C.c_int64)]
+                ("graph_captures", C.c_int64), ("host_admit_ms", C.c_double), ("host_launch_ms", C.c_double),
+                ("host_retire_ms", C.c_double), ("host_finalize_ms", C.c_double), ("host_collect_ms", C.c_double),
+                ("pool_bytes", C.c_uint64)]
 
 
 class gpi_timing(C.Structure):

@@ -326,6 +326,11 @@ typedef struct gpi_server_stats {
     uint64_t hbm_used, hbm_budget;
     int64_t graph_chunks;        /* chunks run as a graph replay */
     int64_t graph_captures;      /* graphs captured and instantiated */
+    /* host time of the scheduler thread (ms): admitting shots (device state, uploads),
+     * launching chunks (table upload, graph capture and launch), retiring shots (record
+     * copies queued), handing records to the caller, draining for memory release */
+    double host_admit_ms, host_launch_ms, host_retire_ms, host_finalize_ms, host_collect_ms;
+    uint64_t pool_bytes;         /* device memory held by the scheduler's pool */
 } gpi_server_stats;
 
 int gpi_server_create(int device, const gpi_server_opts* opts, gpi_server** out);

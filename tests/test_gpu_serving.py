@@ -95,6 +95,7 @@ def test_server_alone_equals_mix_fixed_tiles(gpu_ctx, tile):
     assert st["mean_batch"] > 1.5    # shots did overlap on the device
     # chunks whose batch did not change replayed captured graphs, some graphs more than once
     assert st["graph_captures"] >= 1 and st["graph_chunks"] > st["graph_captures"]
+    assert st["host_admit_ms"] > 0 and st["host_launch_ms"] > 0 and st["host_retire_ms"] > 0
 
 
 def test_server_direct_launch_equals_plain(gpu_ctx):
