@@ -160,6 +160,37 @@ struct ImageSets {
     const float* va[3];
     const float* vb[3];
 };
+// Elastic imaging of the checkpointed gradient (strips.hip).  Per forward step n: the
+// adjoint stresses before the adjoint stress pass (tau~^{n+1}), the adjoint velocities after
+// the adjoint velocity pass (v~^{n+1/2}) and the forward states around the step (tau^n,
+// tau^{n+1}, v^{n-1/2}, v^{n+1/2}).  Accumulated in FP64 on the slab layout:
+//   T = sum (tr tau~)(tr dtau) and P = sum tau~ . dtau over the normal stresses,
+//   S[s] = sum tau~_s dtau_s per shear stress, R[a] = sum v~_a dv_a per velocity.
+struct ElasticImage {
+    int nn = 0, ns = 0, nv = 0;                     // normal, shear, velocity components
+    const float* an[3] = {}; const float* f0n[3] = {}; const float* f1n[3] = {};
+    const float* as[3] = {}; const float* f0s[3] = {}; const float* f1s[3] = {};
+    const float* av[3] = {}; const float* f0v[3] = {}; const float* f1v[3] = {};
+    int shi[3][3] = {};                             // shear s occupies [0, shi) in (z, y, x)
+    int sbase[3][3] = {};                           // its invmu average: cells p + sbase + {0, e1, e2, e1 + e2}
+    int se1[3] = {}, se2[3] = {};                   // axes (0 z, 1 y, 2 x) of e1, e2
+    int vlo[3][3] = {}, vhi[3][3] = {};             // velocity a occupies [vlo, vhi)
+    int vaxis[3] = {};                              // its density pair: cells v - h e_a and v - h e_a + e_a
+    double* T = nullptr;
+    double* P = nullptr;
+    double* S[3] = {};
+    double* R[3] = {};
+};
+void image_elastic(const Model& m, const ElasticImage& E, hipStream_t s);
+// gradients w.r.t. invlambda, invmu and rho on the stress grid from the accumulators
+void elastic_gradient(const Model& m, const ElasticImage& E, double* g_il, double* g_im, double* g_rho,
+                      hipStream_t s);
+// Normal-stress sources inject dtM * a_n into every normal stress (src/fdtd/source.jl:79-119):
+// per target cell, acc += (sum of the adjoint normal stresses) * a_n at forward step n, and at
+// the end the part of the invlambda/invmu gradient that the Delta-tau form leaves out there.
+void source_trace(const DevPoints& pts, int64_t n, const float* const an[3], int nn, double* acc, hipStream_t s);
+void source_correction(const Model& m, const DevPoints& pts, const double* acc, int nn, double* g_il, double* g_im,
+                       hipStream_t s);
 void image_acoustic_exact(const Model& m, const ImageSets& S, double* g_invK, double* g_rho, float dtI, hipStream_t s);
 
 // Born secondary sources (born.hip): scattered state w2 from background state w1 after the

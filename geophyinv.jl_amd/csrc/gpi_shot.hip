@@ -296,7 +296,6 @@ int alloc_store(gpi_ctx* c, Model& m, int64_t nt, Store** out)
 int alloc_checkpoint_store(gpi_ctx* c, Model& m, int64_t nt, int64_t every, Store** out)
 {
     const gpi_grid& g = m.g;
-    if (g.physics != 0) return fail(c, GPI_EINVAL, "checkpointed gradients are acoustic only");
     if (every < 1) return fail(c, GPI_EINVAL, "the checkpoint interval must be positive");
     Store* s = new Store();
     s->model = &m;
@@ -834,6 +833,200 @@ int gpi_adjoint_checkpoint(gpi_ctx* c, uint64_t model, uint64_t store, const gpi
     shape_of(g.ndim, g.order, g.physics, g.n, GPI_P, n3);
     if ((rc = download_pitched_f64(c, m, g_invK, gK, n3))) return rc;
     if (rho && (rc = download_pitched_f64(c, m, g_rho, gR, n3))) return rc;
+    return GPI_OK;
+}
+
+int gpi_adjoint_checkpoint_elastic(gpi_ctx* c, uint64_t model, uint64_t store, const gpi_adjoint_shot* shot,
+                                   double* g_invlambda, double* g_invmu, double* g_rho)
+{
+    if (!c || !shot || !g_invlambda || !g_invmu || !g_rho) return fail(c, GPI_EINVAL, "null argument");
+    Model* mp = find_model(c, model);
+    if (!mp) return fail(c, GPI_ENOTFOUND, "unknown model handle");
+    Store* sp = find_store(c, store);
+    if (!sp || sp->model != mp) return fail(c, GPI_ENOTFOUND, "unknown store handle for this model");
+    if (sp->every < 1 || !sp->replay) return fail(c, GPI_EINVAL, "the store holds no checkpoints");
+    Model& m = *mp;
+    const Store& sv = *sp;
+    const Replay& R = *sv.replay;
+    const gpi_grid& g = m.g;
+    if (g.physics != 1) return fail(c, GPI_EINVAL, "the elastic gradient needs an elastic model");
+    if (!m.coef[GPI_C_INVLAMBDA] || !m.coef[GPI_C_INVMU]) return fail(c, GPI_EINVAL, "the model lacks invlambda/invmu");
+    const int64_t nt = shot->nt;
+    if (nt != sv.nt) return fail(c, GPI_EINVAL, "nt differs from the stored forward run");
+    if (shot->nsrc2 < 0 || shot->nsrc2 > 8 || (shot->nsrc2 > 0 && !shot->src2))
+        return fail(c, GPI_EINVAL, "bad number of adjoint source fields");
+    for (int q = 0; q < shot->nsrc2; ++q)   // the adjoint of velocity records only
+        if (!is_velocity(shot->src2[q].field))
+            return fail(c, GPI_EINVAL, "elastic gradients take velocity records only (adjoint sources on vx, vy, vz)");
+    HIPTRY(c, hipSetDevice(c->device));
+    const bool d3 = g.ndim == 3;
+    const int H = m.h;
+    const State& s1 = *m.st0;
+    Scratch S(c);
+    struct Held {
+        gpi_ctx* c;
+        State st;
+        ~Held()
+        {
+            (void)hipStreamSynchronize(c->stream);
+            free_allocs(c, st.allocs, st.bytes);
+        }
+    } h2{c, State{}};
+    int rc = alloc_state(c, m, &h2.st);
+    if (rc) return rc;
+    const State& s2 = h2.st;
+    PassExtras XV2{}, XS2{};
+    Sources src2;
+    if ((rc = plan_sources(c, m, S, shot->nsrc2, shot->src2, shot->wav2, nt, true, XV2, XS2, src2))) return rc;
+    PassExtras XV1 = R.xv, XS1 = R.xs;
+
+    // components, their regions on the slab (z, y, x) and their coefficient stencils
+    const int nz = (int)g.n[0], ny = d3 ? (int)g.n[1] : 1, nx = (int)g.n[2];
+    const std::vector<int> nf = d3 ? std::vector<int>{GPI_TAUXX, GPI_TAUYY, GPI_TAUZZ}
+                                   : std::vector<int>{GPI_TAUXX, GPI_TAUZZ};
+    const std::vector<int> sf = d3 ? std::vector<int>{GPI_TAUXY, GPI_TAUXZ, GPI_TAUYZ} : std::vector<int>{GPI_TAUXZ};
+    const std::vector<int> vf = d3 ? std::vector<int>{GPI_VX, GPI_VY, GPI_VZ} : std::vector<int>{GPI_VX, GPI_VZ};
+    ElasticImage E{};
+    E.nn = (int)nf.size();
+    E.ns = (int)sf.size();
+    E.nv = (int)vf.size();
+    for (int q = 0; q < E.ns; ++q) {
+        // stress grids of str3d_elastic / str_el_point and the @av_* cells of dtav*imu
+        int* hi = E.shi[q];
+        int* b = E.sbase[q];
+        if (!d3) {                        // tauxz: (nz-h) x (nx-h), cells (i..i+1, k..k+1)
+            hi[0] = nz - H, hi[1] = 1, hi[2] = nx - H;
+            E.se1[q] = 0, E.se2[q] = 2;
+        } else if (sf[q] == GPI_TAUXY) {  // dvxdy grid, cells (i+h, j..j+1, k..k+1)
+            hi[0] = nz - 2 * H, hi[1] = ny - H, hi[2] = nx - H;
+            b[0] = H;
+            E.se1[q] = 1, E.se2[q] = 2;
+        } else if (sf[q] == GPI_TAUXZ) {  // dvxdz grid, cells (i..i+1, j+h, k..k+1)
+            hi[0] = nz - H, hi[1] = ny - 2 * H, hi[2] = nx - H;
+            b[1] = H;
+            E.se1[q] = 0, E.se2[q] = 2;
+        } else {                          // dvydz grid, cells (i..i+1, j..j+1, k+h)
+            hi[0] = nz - H, hi[1] = ny - H, hi[2] = nx - 2 * H;
+            b[2] = H;
+            E.se1[q] = 0, E.se2[q] = 1;
+        }
+    }
+    for (int q = 0; q < E.nv; ++q) {
+        // vel*_point updates v[inn]: [h, n-h) across, [h, n) along its own axis
+        const int ax = vf[q] == GPI_VX ? 2 : vf[q] == GPI_VY ? 1 : 0;
+        const int n3[3] = {nz, ny, nx};
+        for (int a = 0; a < 3; ++a) {
+            E.vlo[q][a] = (a == 1 && !d3) ? 0 : H;
+            E.vhi[q][a] = (a == 1 && !d3) ? 1 : (a == ax ? n3[a] : n3[a] - H);
+        }
+        E.vaxis[q] = ax;
+    }
+    const int64_t C = sv.every;
+    const int64_t L = std::min(C, nt);
+    const size_t slab = (size_t)m.slab;
+    const size_t bytes = sizeof(float) * slab;
+    // segment buffers: the forward stresses tau^{n0} .. tau^{n1} and velocities v^{n0-1/2} .. v^{n1-1/2}
+    std::vector<int> all = nf;
+    all.insert(all.end(), sf.begin(), sf.end());
+    all.insert(all.end(), vf.begin(), vf.end());
+    std::vector<float*> seg(all.size(), nullptr);
+    for (float*& p : seg)
+        if ((rc = S.alloc(&p, (size_t)(L + 1) * slab))) return rc;
+    double* acc[8] = {};
+    const int nacc = 2 + E.ns + E.nv;
+    for (int q = 0; q < nacc; ++q) {
+        if ((rc = S.alloc(&acc[q], slab))) return rc;
+        HIPTRY(c, hipMemsetAsync(acc[q], 0, sizeof(double) * slab, c->stream));
+    }
+    E.T = acc[0];
+    E.P = acc[1];
+    for (int q = 0; q < E.ns; ++q) E.S[q] = acc[2 + q];
+    for (int q = 0; q < E.nv; ++q) E.R[q] = acc[2 + E.ns + q];
+    if ((rc = zero_state(c, m, s2, c->stream))) return rc;
+    // normal-stress sources of the forward run: their per-target traces (source_trace)
+    std::vector<int> nsrc;
+    std::vector<double*> sacc;
+    for (size_t q = 0; q < R.src.pts.size(); ++q) {
+        const int f = R.src.pts[q].field;
+        if ((f != GPI_TAUXX && f != GPI_TAUYY && f != GPI_TAUZZ) || R.src.pts[q].ntarget <= 0) continue;
+        double* a = nullptr;
+        if ((rc = S.alloc(&a, (size_t)R.src.pts[q].ntarget))) return rc;
+        HIPTRY(c, hipMemsetAsync(a, 0, sizeof(double) * (size_t)R.src.pts[q].ntarget, c->stream));
+        nsrc.push_back((int)q);
+        sacc.push_back(a);
+    }
+    auto save = [&](int64_t slot) -> int {
+        for (size_t q = 0; q < all.size(); ++q)
+            HIPTRY(c, hipMemcpyAsync(seg[q] + (size_t)slot * slab, s1.field[all[q]], bytes, hipMemcpyDeviceToDevice,
+                                     c->stream));
+        return GPI_OK;
+    };
+
+    Sampler smp(c, nt);
+    int64_t launches = 0, it = 0;
+    const int64_t K = (nt + C - 1) / C;
+    HIPTRY(c, hipEventRecord(c->ev[0], c->stream));
+    for (int64_t k = K - 1; k >= 0; --k) {
+        const int64_t n0 = k * C, n1 = std::min(n0 + C, nt);
+        if (k == 0) rc = zero_state(c, m, s1, c->stream);
+        else rc = copy_state(c, m, s1, sv.ckpt[k - 1], false, c->stream);
+        if (rc || (rc = save(0))) return rc;
+        for (int64_t n = n0; n < n1; ++n) {
+            XV1.it = XS1.it = n;
+            if ((rc = velocity_pass(c, m, s1, XV1))) return rc;
+            for (int q : R.src.vfall) inject(m, s1, R.src.pts[q], n, c->stream);
+            if ((rc = stress_pass(c, m, s1, XS1))) return rc;
+            for (int q : R.src.sfall) inject(m, s1, R.src.pts[q], n, c->stream);
+            if ((rc = save(n - n0 + 1))) return rc;
+            launches += 2 + (int64_t)(R.src.vfall.size() + R.src.sfall.size());
+        }
+        // adjoint steps of this segment (forward step n = nt - 1 - it), imaging between the passes
+        for (int64_t n = n1 - 1; n >= n0; --n, ++it) {
+            hipEvent_t* Ev = smp.at(it);
+            XV2.it = XS2.it = it;
+            if (Ev) HIPTRY(c, hipEventRecord(Ev[0], c->stream));
+            if ((rc = velocity_pass(c, m, s2, XV2))) return rc;
+            if (Ev) HIPTRY(c, hipEventRecord(Ev[1], c->stream));
+            for (int q : src2.vfall) inject(m, s2, src2.pts[q], it, c->stream);
+            const size_t a = (size_t)(n - n0) * slab, b = a + slab;
+            int w = 0;
+            for (int q = 0; q < E.nn; ++q, ++w) {
+                E.an[q] = s2.field[nf[q]];
+                E.f0n[q] = seg[w] + a;
+                E.f1n[q] = seg[w] + b;
+            }
+            for (int q = 0; q < E.ns; ++q, ++w) {
+                E.as[q] = s2.field[sf[q]];
+                E.f0s[q] = seg[w] + a;
+                E.f1s[q] = seg[w] + b;
+            }
+            for (int q = 0; q < E.nv; ++q, ++w) {
+                E.av[q] = s2.field[vf[q]];
+                E.f0v[q] = seg[w] + a;
+                E.f1v[q] = seg[w] + b;
+            }
+            image_elastic(m, E, c->stream);
+            for (size_t q = 0; q < nsrc.size(); ++q) source_trace(R.src.pts[nsrc[q]], n, E.an, E.nn, sacc[q], c->stream);
+            if (Ev) HIPTRY(c, hipEventRecord(Ev[2], c->stream));
+            if ((rc = stress_pass(c, m, s2, XS2))) return rc;
+            for (int q : src2.sfall) inject(m, s2, src2.pts[q], it, c->stream);
+            if (Ev) HIPTRY(c, hipEventRecord(Ev[3], c->stream));
+            launches += 3 + (int64_t)(src2.vfall.size() + src2.sfall.size());
+            smp.done(it);
+        }
+    }
+    if ((rc = finish_timing(c, smp, nt, launches))) return rc;
+    double* gout[3] = {};
+    for (auto& p : gout)
+        if ((rc = S.alloc(&p, slab))) return rc;
+    elastic_gradient(m, E, gout[0], gout[1], gout[2], c->stream);
+    for (size_t q = 0; q < nsrc.size(); ++q) source_correction(m, R.src.pts[nsrc[q]], sacc[q], E.nn, gout[0], gout[1], c->stream);
+    HIPTRY(c, hipGetLastError());
+    int64_t n3[3];
+    shape_of(g.ndim, g.order, g.physics, g.n, GPI_TAUXX, n3);
+    double* host[3] = {g_invlambda, g_invmu, g_rho};
+    for (int q = 0; q < 3; ++q)
+        if ((rc = download_pitched_f64(c, m, host[q], gout[q], n3))) return rc;
     return GPI_OK;
 }
 

@@ -152,7 +152,194 @@ __global__ __launch_bounds__(256) void image_exact_kernel(ImageExactParams P)
             if (S.n2[1 + a]) S.n2[1 + a][o] = S.c2[1 + a][o];
 }
 
+// ---- elastic imaging ----------------------------------------------------------------
+// Derivation (interior, no CPML): with tau -= C E v and v -= B D tau, the Lagrange
+// multipliers (Lambda of the stress updates, M of the velocity updates) satisfy the same
+// scheme run backwards in time with tau~ = -C Lambda and v~ = B M, because E^T = -D on the
+// staggered grids.  Then dL/dC = -Lambda^{n+1} . E v^{n+1/2} with E v = -C^{-1} dtau and
+// dL/dB = -M . D tau with D tau = -B^{-1} dv, which the chain rule through
+// dtM = dt(1/invlambda + 2/invmu), dtlambda = dt/invlambda, dtav*mu = dt/avg4(invmu) and
+// dtinvav*irho = dt/avg2(rho) (src/fdtd/medium.jl:143-186) turns into the sums of
+// ElasticImage: with d normal stresses, a = M + (d-1)L and b = M - L the eigenvalues of
+// the normal block, Lambda . E v = T/(d a^2) + (P - T/d)/b^2.  The adjoint sources are
+// the reference's 2 (dobs - dcal) = -dJ/dv (database.jl:371-380), so the adjoint run holds
+// -(tau~, v~) and every sum enters negated.
+
+struct ElasticParams {
+    ElasticImage E;
+    int nz, ny, nx, ld, py, h;
+    int64_t slab;
+};
+
+__device__ __forceinline__ void slab_coords(int64_t o, int ld, int py, int c[3])
+{
+    c[0] = (int)(o % ld);
+    const int64_t r = o / ld;
+    c[1] = (int)(r % py);
+    c[2] = (int)(r / py);
+}
+
+__global__ __launch_bounds__(256) void image_elastic_kernel(ElasticParams P)
+{
+    const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (o >= P.slab) return;
+    int c[3];
+    slab_coords(o, P.ld, P.py, c);
+    const ElasticImage& E = P.E;
+    if (c[0] < P.nz && c[1] < P.ny && c[2] < P.nx) {
+        double ta = 0.0, td = 0.0, dot = 0.0;
+        for (int q = 0; q < E.nn; ++q) {
+            const double t = (double)E.an[q][o];
+            const double d = (double)E.f1n[q][o] - (double)E.f0n[q][o];
+            ta += t;
+            td += d;
+            dot += t * d;
+        }
+        E.T[o] += ta * td;
+        E.P[o] += dot;
+    }
+    for (int q = 0; q < E.ns; ++q)
+        if (c[0] < E.shi[q][0] && c[1] < E.shi[q][1] && c[2] < E.shi[q][2])
+            E.S[q][o] += (double)E.as[q][o] * ((double)E.f1s[q][o] - (double)E.f0s[q][o]);
+    for (int q = 0; q < E.nv; ++q)
+        if (c[0] >= E.vlo[q][0] && c[0] < E.vhi[q][0] && c[1] >= E.vlo[q][1] && c[1] < E.vhi[q][1] &&
+            c[2] >= E.vlo[q][2] && c[2] < E.vhi[q][2])
+            E.R[q][o] += (double)E.av[q][o] * ((double)E.f1v[q][o] - (double)E.f0v[q][o]);
+}
+
+__global__ __launch_bounds__(256) void elastic_gradient_kernel(ElasticParams P, const float* __restrict__ il,
+                                                               const float* __restrict__ im, double dt,
+                                                               double* __restrict__ gl, double* __restrict__ gm,
+                                                               double* __restrict__ gr)
+{
+    const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (o >= P.slab) return;
+    int c[3];
+    slab_coords(o, P.ld, P.py, c);
+    if (!(c[0] < P.nz && c[1] < P.ny && c[2] < P.nx)) return;
+    const ElasticImage& E = P.E;
+    const int64_t st[3] = {1, P.ld, (int64_t)P.ld * P.py};
+    const double d = (double)E.nn;
+    const double L = (double)il[o], M = (double)im[o];   // invlambda, invmu
+    const double T = E.T[o], Pn = E.P[o];
+    // a = dt (d/invlambda + 2/invmu), b = 2 dt/invmu
+    const double ra = d + 2.0 * L / M;       // a invlambda / dt
+    const double rb = d * M / L + 2.0;       // a invmu / dt
+    gl[o] = -T / (dt * ra * ra);
+    double g = 2.0 * T / (d * dt * rb * rb) + (Pn - T / d) / (2.0 * dt);
+    // shear: each shear point spreads its term over the four cells of its invmu average
+    double sh = 0.0;
+    for (int q = 0; q < E.ns; ++q)
+        for (int k = 0; k < 4; ++k) {
+            int p[3] = {c[0] - E.sbase[q][0], c[1] - E.sbase[q][1], c[2] - E.sbase[q][2]};
+            if (k & 1) --p[E.se1[q]];
+            if (k & 2) --p[E.se2[q]];
+            if (p[0] < 0 || p[1] < 0 || p[2] < 0 || p[0] >= E.shi[q][0] || p[1] >= E.shi[q][1] ||
+                p[2] >= E.shi[q][2])
+                continue;
+            sh += E.S[q][p[0] + st[1] * p[1] + st[2] * p[2]];
+        }
+    gm[o] = -(g + sh / (4.0 * dt));
+    // density: each velocity point spreads over the two cells of its rho average
+    double r = 0.0;
+    for (int q = 0; q < E.nv; ++q)
+        for (int off = P.h; off >= P.h - 1; --off) {
+            int v[3] = {c[0], c[1], c[2]};
+            v[E.vaxis[q]] += off;
+            if (v[0] < E.vlo[q][0] || v[0] >= E.vhi[q][0] || v[1] < E.vlo[q][1] || v[1] >= E.vhi[q][1] ||
+                v[2] < E.vlo[q][2] || v[2] >= E.vhi[q][2])
+                continue;
+            r += E.R[q][v[0] + st[1] * v[1] + st[2] * v[2]];
+        }
+    gr[o] = r / (2.0 * dt);
+}
+
+// The normal-stress source of forward step n at each target: a_n = sum of weight x wavelet
+// (the injection sum of the stress pass), times the trace of the adjoint normal stresses.
+__global__ __launch_bounds__(256) void source_trace_kernel(const int64_t* __restrict__ toff,
+                                                           const int32_t* __restrict__ tptr,
+                                                           const int32_t* __restrict__ tsrc,
+                                                           const float* __restrict__ tw, const float* __restrict__ wav,
+                                                           int npts, int ntarget, int64_t n, const float* t0,
+                                                           const float* t1, const float* t2, double* acc)
+{
+    const int t = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (t >= ntarget) return;
+    float a = 0.0f;
+    for (int k = tptr[t]; k < tptr[t + 1]; ++k) a = a + tw[k] * wav[n * npts + tsrc[k]];
+    const int64_t c = toff[t];
+    double tr = (double)t0[c] + (double)t1[c];
+    if (t2) tr += (double)t2[c];
+    acc[t] += tr * (double)a;
+}
+
+// With a = M + (d-1)L the trace eigenvalue of the normal block and Q = acc (adjoint run
+// fields): the exact gradient adds (dt/il^2)(d-1)(M-L)/a^2 Q to invlambda and
+// -(2dt/im^2)(d-1)L/a^2 Q to invmu at the target (the source term's own dependence on dtM
+// and the Delta-tau form's source share).
+__global__ __launch_bounds__(256) void source_correction_kernel(const int64_t* __restrict__ toff, int ntarget,
+                                                                const double* __restrict__ acc,
+                                                                const float* __restrict__ il,
+                                                                const float* __restrict__ im, double dt, double d,
+                                                                double* gl, double* gm)
+{
+    const int t = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (t >= ntarget) return;
+    const int64_t c = toff[t];
+    const double L0 = (double)il[c], M0 = (double)im[c];
+    const double L = dt / L0, M = dt * (1.0 / L0 + 2.0 / M0), a = M + (d - 1.0) * L;
+    gl[c] += dt / (L0 * L0) * (d - 1.0) * (M - L) / (a * a) * acc[t];
+    gm[c] -= 2.0 * dt / (M0 * M0) * (d - 1.0) * L / (a * a) * acc[t];
+}
+
+ElasticParams elastic_params(const Model& m, const ElasticImage& E)
+{
+    ElasticParams P{};
+    P.E = E;
+    P.nz = (int)m.g.n[0];
+    P.ny = m.g.ndim == 3 ? (int)m.g.n[1] : 1;
+    P.nx = (int)m.g.n[2];
+    P.ld = (int)m.pitch0;
+    P.py = m.g.ndim == 3 ? (int)m.pitch1 : 1;
+    P.h = m.h;
+    P.slab = (int64_t)m.slab;
+    return P;
+}
+
 }  // namespace
+
+void image_elastic(const Model& m, const ElasticImage& E, hipStream_t s)
+{
+    const ElasticParams P = elastic_params(m, E);
+    hipLaunchKernelGGL(image_elastic_kernel, dim3((unsigned)((P.slab + 255) / 256)), dim3(256), 0, s, P);
+}
+
+void elastic_gradient(const Model& m, const ElasticImage& E, double* g_il, double* g_im, double* g_rho,
+                      hipStream_t s)
+{
+    const ElasticParams P = elastic_params(m, E);
+    hipLaunchKernelGGL(elastic_gradient_kernel, dim3((unsigned)((P.slab + 255) / 256)), dim3(256), 0, s, P,
+                       (const float*)m.coef[GPI_C_INVLAMBDA], (const float*)m.coef[GPI_C_INVMU], (double)m.g.dt,
+                       g_il, g_im, g_rho);
+}
+
+void source_trace(const DevPoints& pts, int64_t n, const float* const an[3], int nn, double* acc, hipStream_t s)
+{
+    if (pts.ntarget <= 0) return;
+    hipLaunchKernelGGL(source_trace_kernel, dim3((unsigned)((pts.ntarget + 255) / 256)), dim3(256), 0, s,
+                       (const int64_t*)pts.toff, (const int32_t*)pts.tptr, (const int32_t*)pts.tsrc,
+                       (const float*)pts.tw, (const float*)pts.wav, pts.npts, pts.ntarget, n, an[0], an[1],
+                       nn > 2 ? an[2] : (const float*)nullptr, acc);
+}
+
+void source_correction(const Model& m, const DevPoints& pts, const double* acc, int nn, double* g_il, double* g_im,
+                       hipStream_t s)
+{
+    if (pts.ntarget <= 0) return;
+    hipLaunchKernelGGL(source_correction_kernel, dim3((unsigned)((pts.ntarget + 255) / 256)), dim3(256), 0, s,
+                       (const int64_t*)pts.toff, pts.ntarget, acc, (const float*)m.coef[GPI_C_INVLAMBDA],
+                       (const float*)m.coef[GPI_C_INVMU], (double)m.g.dt, (double)nn, g_il, g_im);
+}
 
 void image_acoustic_exact(const Model& m, const ImageSets& S, double* g_invK, double* g_rho, float dtI, hipStream_t s)
 {

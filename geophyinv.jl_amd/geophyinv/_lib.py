@@ -128,6 +128,8 @@ SIGNATURES = {
                                          C.POINTER(C.c_uint64)]),
     "gpi_adjoint_checkpoint": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(gpi_adjoint_shot), C.c_float,
                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "gpi_adjoint_checkpoint_elastic": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(gpi_adjoint_shot),
+                                                 C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "gpi_last_timing": (C.c_int, [C.c_void_p, C.POINTER(gpi_timing)]),
     "gpi_set_pass_sampling": (C.c_int, [C.c_void_p, C.c_int32]),
 }

@@ -163,8 +163,10 @@ class SeisForwExpt:
         # checkpoint > 0: :forward_save keeps the whole state every `checkpoint` steps and
         # :adjoint recomputes the forward wavefield exactly from it (gradients in FP64, 2-D
         # and 3-D acoustic); 0: the reference's boundary strips and time reversal
-        if checkpoint < 0 or (checkpoint and physics != 0):
-            raise ValueError("checkpoint must be >= 0, and checkpointed gradients are acoustic only")
+        if checkpoint < 0:
+            raise ValueError("checkpoint must be >= 0")
+        if checkpoint and physics == 1 and any(f not in ("vx", "vy", "vz") for f in rfields):
+            raise ValueError("elastic gradients take velocity records only (rfields vx, vy, vz)")
         self.checkpoint = int(checkpoint)
         self.grad_rho = bool(grad_rho)
         self.born = bool(attrib_mod.born)
@@ -498,11 +500,12 @@ class SeisForwExpt:
         return dict(zip(names, outs)), finals
 
     def run_adjoint(self, iss: int):
-        """:adjoint run of supersource ``iss`` (npw = 2, 2-D acoustic): returns its
-        gradients (``pap[1].ss[iss].gradients``) w.r.t. invK and rho on the padded grid."""
-        if self.physics != 0 or (self.ndim != 2 and not self.checkpoint):
+        """:adjoint run of supersource ``iss`` (npw = 2): returns its gradients
+        (``pap[1].ss[iss].gradients``) on the padded grid, w.r.t. invK and rho (acoustic) or
+        invlambda, invmu and rho (elastic, checkpoint > 0)."""
+        if (self.physics != 0 or self.ndim != 2) and not self.checkpoint:
             raise NotImplementedError("the reference gradient exists for 2-D acoustic only (gradient.jl:17-64); "
-                                      "3-D acoustic gradients need checkpoint > 0")
+                                      "3-D and elastic gradients need checkpoint > 0")
         if iss not in self._stores:
             raise RuntimeError("run the :forward_save mode before the :adjoint mode")
         ctx = self._ctx
@@ -515,6 +518,13 @@ class SeisForwExpt:
         sh.nsrc1, sh.src1, sh.wav1 = n1, s1, w1
         sh.nsrc2, sh.src2, sh.wav2 = n2, s2, w2
         dtI = F32(1.0 / self.tgrid.step)   # fc[:dtI] (fdtd.jl:326)
+        if self.physics == 1:
+            out = {k: np.zeros(self.ext_shape, np.float64, order="F") for k in ("invlambda", "invmu", "rho")}
+            ctx.check(ctx.lib.gpi_adjoint_checkpoint_elastic(ctx.h, self._model, self._stores[iss], C.byref(sh),
+                                                             *[_lib.dptr(out[k]) for k in ("invlambda", "invmu",
+                                                                                           "rho")]))
+            self.timing = ctx.timing()
+            return out
         if self.checkpoint:
             gk = np.zeros(self.ext_shape, np.float64, order="F")
             gr = np.zeros(self.ext_shape, np.float64, order="F") if self.grad_rho else None

@@ -78,3 +78,24 @@ def backprop(g, m, loss, dobs, pa, mparams=None) -> None:
         gm32 = (gm.astype(np.float64) * np.exp(xr) * np.float64(pa.ref_mod[name])).astype(F32)
         out.append(gm32.ravel(order="F"))
     g[...] = np.concatenate(out)
+
+
+def chainrule_elastic(g_invlambda, g_invmu, g_rho, vp, vs, rho):
+    """Gradients w.r.t. (vp, vs, rho) from gradients w.r.t. the independent elastic
+    parameters (invlambda, invmu, rho) of the same cells, FP64.
+
+    The elastic extension of old/chainrule.jl:43-45 (``grad_of_vp``/``grad_of_rho`` for
+    invK = 1/(vp^2 rho)): with invlambda = 1/((vp^2 - 2 vs^2) rho) and invmu = 1/(vs^2 rho)
+    (src/media/media.jl:111-130),
+      d invlambda/d vp = -2 vp rho invlambda^2,  d invlambda/d vs = 4 vs rho invlambda^2,
+      d invlambda/d rho = -invlambda/rho,        d invmu/d vs = -2 vs rho invmu^2,
+      d invmu/d rho = -invmu/rho.
+    Returns (g_vp, g_vs, g_rho)."""
+    vp, vs, rho = (np.asarray(a, np.float64) for a in (vp, vs, rho))
+    gl, gm, gr = (np.asarray(a, np.float64) for a in (g_invlambda, g_invmu, g_rho))
+    il = 1.0 / ((vp * vp - 2.0 * vs * vs) * rho)
+    im = 1.0 / (vs * vs * rho)
+    g_vp = gl * (-2.0 * vp * rho * il * il)
+    g_vs = gl * (4.0 * vs * rho * il * il) + gm * (-2.0 * vs * rho * im * im)
+    g_rho = gr + gl * (-il / rho) + gm * (-im / rho)
+    return g_vp, g_vs, g_rho

@@ -235,7 +235,7 @@ int gpi_adjoint(gpi_ctx* ctx, uint64_t model, uint64_t store, const gpi_adjoint_
                 float* g_invK, float* g_rho);
 int gpi_store_free(gpi_ctx* ctx, uint64_t store);
 
-/* ---- checkpointed gradient (acoustic, 2-D and 3-D) ----------------------------------- */
+/* ---- checkpointed gradient (2-D and 3-D; acoustic and elastic) ----------------------- */
 /* The :forward_save run of gpi_forward_save with checkpoints in place of boundary strips
  * (no reference counterpart: the reference keeps strips, src/fdtd/boundary.jl:217-306, and
  * has no 3-D gradient).  The store keeps the whole wavefield and CPML state after every
@@ -260,6 +260,20 @@ int gpi_forward_checkpoint(gpi_ctx* ctx, uint64_t model, const gpi_shot* shot, i
  * NULL (the velocities are then not kept: 1 instead of 1 + ndim slabs per segment step). */
 int gpi_adjoint_checkpoint(gpi_ctx* ctx, uint64_t model, uint64_t store, const gpi_adjoint_shot* shot,
                            float dtI, double* g_invK, double* g_rho);
+
+/* Elastic gradient of one supersource from a checkpoint store (2-D P-SV and 3-D; no
+ * reference counterpart: the reference's gradient is acoustic only, src/fdtd/gradient.jl).
+ * The store comes from gpi_forward_checkpoint on an elastic model whose receivers record
+ * velocities only; the adjoint sources (src2/wav2) are the time-reversed velocity
+ * residuals, as in gpi_adjoint_checkpoint.  The adjoint wavefield is the forward scheme run
+ * backwards, which is the exact adjoint of the discrete update away from the CPML; per
+ * forward step n the adjoint stresses tau~^{n+1} and velocities v~^{n+1/2} meet the forward
+ * increments tau^{n+1} - tau^n and v^{n+1/2} - v^{n-1/2}, and the chain rule through the
+ * dt-scaled coefficients of update_dmod! (src/fdtd/medium.jl:143-186) gives, in FP64, the
+ * gradients w.r.t. the independent parameters invlambda, invmu and rho (mod, medium.jl:81-95)
+ * on the stress grid.  Memory: per segment step, every stress and velocity slab. */
+int gpi_adjoint_checkpoint_elastic(gpi_ctx* ctx, uint64_t model, uint64_t store, const gpi_adjoint_shot* shot,
+                                   double* g_invlambda, double* g_invmu, double* g_rho);
 
 /* ---- Born modelling (acoustic, 2-D and 3-D) ------------------------------------------ */
 /* One supersource of FdtdAcoustic{Born} in :forward mode (src/fdtd/born.jl,

@@ -97,9 +97,13 @@ def test_checkpoint_matches_reference_flow_in_the_interior(gpu_ctx):
         assert e < 1e-5
 
 
-def test_checkpoint_rejects_elastic(gpu_ctx):
+def test_elastic_checkpoint_needs_velocity_records(gpu_ctx):
+    """Elastic gradients (tests/test_gpu_elastic_gradient.py) take velocity records only: the
+    adjoint of a stress or p record is not a velocity source."""
     from helpers import layered_elastic
+    kw = dict(medium=layered_elastic(), tgrid=G.Grid(0.0, 1e-4, 10),
+              ageom=[G.AGeomss({"z": [50.0], "x": [100.0]}, {"z": [0.0], "x": [100.0]})],
+              srcwav=[G.Srcs(1, G.Grid(0.0, 1e-4, 10), ["vz"])], checkpoint=10, context=gpu_ctx)
     with pytest.raises(ValueError):
-        G.SeisForwExpt(G.FdtdElastic("forward_save", 2), medium=layered_elastic(), tgrid=G.Grid(0.0, 1e-4, 10),
-                       ageom=[G.AGeomss({"z": [50.0], "x": [100.0]}, {"z": [0.0], "x": [100.0]})],
-                       srcwav=[G.Srcs(1, G.Grid(0.0, 1e-4, 10), ["vz"])], checkpoint=10, context=gpu_ctx)
+        G.SeisForwExpt(G.FdtdElastic("forward_save", 2), rfields=["p", "vz"], **kw)
+    G.SeisForwExpt(G.FdtdElastic("forward_save", 2), rfields=["vx", "vz"], **kw).free()
