@@ -271,7 +271,83 @@ def run_gradient3d(args, G, _lib, gallery):
     return line
 
 
-def fd_check_3d(args, pa, m, g, J, dobs, loss, mp, start, G):
+def run_gradient3d_elastic(args, G, _lib, gallery):
+    """The 3-D elastic FWI gradient: a (n+40)^3 three-layer medium at 5 m with a dipping
+    interface, an explosive 25 Hz source, vx/vy/vz on a 64 x 64 surface patch from the true
+    model, the gradient w.r.t. log(invlambda, invmu, rho) at the start model (interface one
+    cell shallower), checkpoints every --checkpoint steps, FP64 accumulation.  One bench
+    step is one whole gradient (forward run with checkpoints, residuals and adjoint sources
+    on the host, adjoint run with recomputation)."""
+    import numpy as np
+
+    nt = args.nt or 2000
+    n = args.grid_n if args.grid_n != 400 else 256
+    t0 = time.perf_counter()
+    true, start, ageom, srcwav, tgrid, kw = gallery.elastic_fwi_3d(n=n, nt=nt)
+    ctx = _lib.Context(0)
+    pt = G.SeisForwExpt(G.FdtdElastic(), medium=true, tgrid=tgrid, ageom=ageom, srcwav=srcwav, context=ctx, **kw)
+    pt.update()
+    dobs = pt.data
+    pt.free()
+    pa = G.SeisForwExpt(G.FdtdElastic("forward_save", 2), medium=start, tgrid=tgrid, ageom=ageom, srcwav=srcwav,
+                        context=ctx, checkpoint=args.checkpoint, **kw)
+    setup_s = time.perf_counter() - t0
+    mp = ["invlambda", "invmu", "rho"]
+    loss = G.L2DistLoss()
+    m = pa.model_vector(mp)
+    g = np.zeros_like(m)
+    for _ in range(args.warmup):
+        G.gradient(g, m, loss, dobs, pa, mp)
+    wall, adj_ms = [], []
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        G.gradient(g, m, loss, dobs, pa, mp)
+        wall.append(time.perf_counter() - t1)
+        adj_ms.append(pa.timing["total_ms"])
+    J = G.lossvalue(m, loss, dobs, pa, mp)
+    fwd_ms = pa.timing["total_ms"]
+    shape = pa.ext_shape
+    cells = int(np.prod(shape))
+    mean_wall = float(np.mean(wall))
+    K = (nt + args.checkpoint - 1) // args.checkpoint
+    slab = 4 * (-(-(shape[0] + 1) // 32) * 32) * (shape[1] + 1) * (shape[2] + 1)
+    line = {
+        "metric": "Gpt-updates/s (3-D elastic FWI gradient, checkpointed, FP32 wavefields, FP64 reductions)",
+        "value": round(3 * cells * nt / mean_wall / 1e9, 3),
+        "unit": "Gpt-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(mean_wall * 1e3, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"3-D elastic FWI gradient w.r.t. log(invlambda, invmu, rho): {n}^3 at 5 m, "
+                               f"20-cell CPML ({shape[0]}^3), three layers vp 1800/2600/3500 m/s with a seeded "
+                               "dipping interface, start = interface one cell shallower, 25 Hz explosive source, "
+                               f"64x64 surface receivers (vx, vy, vz), checkpoints every {args.checkpoint} steps",
+                   "time_steps": nt, "grid": list(shape), "order": pa.order},
+        "propagations_per_gradient": 3,
+        "gradient_s": round(mean_wall, 2),
+        "forward_device_ms": round(fwd_ms, 1),
+        "adjoint_device_ms": round(float(np.mean(adj_ms)), 1),
+        "misfit": J,
+        "gradient_norms": {k: float(np.linalg.norm(x)) for k, x in zip(mp, np.split(g.astype(np.float64), 3))},
+        "checkpoint_gb": round((K - 1) * state_floats_3d_elastic(shape, pa.npml) * 4 / 1e9, 1),
+        "segment_gb": round((args.checkpoint + 1) * 9 * slab / 1e9, 1),
+        "setup_s": round(setup_s, 1),
+    }
+    if args.fd_cells:
+        line["fd_check"] = fd_check_3d(args, pa, m, g, J, dobs, loss, mp, start, G, blocks=(0, 1, 2))
+    ctx.close()
+    return line
+
+
+def state_floats_3d_elastic(shape, npml):
+    """Floats of one 3-D elastic state: nine wavefield slabs and the CPML memory of the
+    twelve derivative fields (two strips of npml per derivative along its own axis)."""
+    nz, ny, nx = shape
+    slab = (-(-(nz + 1) // 32) * 32) * (ny + 1) * (nx + 1)
+    return 9 * slab + 12 * 2 * npml * max(nz * ny, ny * nx, nz * nx)
+
+
+def fd_check_3d(args, pa, m, g, J, dobs, loss, mp, start, G, blocks=(0,)):
     """Finite differences of the misfit along Gaussian bumps (radius 3 cells) of log(invK)
     centred on seeded interior cells, against the gradient's directional derivative.
     Fourth-order central differences from plain forward runs at m +- eps*b, m +- 2 eps*b;
@@ -290,12 +366,14 @@ def fd_check_3d(args, pa, m, g, J, dobs, loss, mp, start, G):
     pa.attrib_mod.mode = "forward"
     out = []
     w = 9
-    for c in centres:
+    nb = b_len = int(np.prod(n))
+    for c, blk in ((c, k) for c in centres for k in blocks):
         b = np.zeros(n, np.float64, order="F")
         sl = tuple(slice(int(ci) - w, int(ci) + w + 1) for ci in c)
         zz, yy, xx = np.meshgrid(*[np.arange(-w, w + 1)] * 3, indexing="ij")
         b[sl] = np.exp(-(zz ** 2 + yy ** 2 + xx ** 2) / (2.0 * 3.0 ** 2))
-        bv = b.ravel(order="F")
+        bv = np.zeros(m.size)
+        bv[blk * nb:blk * nb + b_len] = b.ravel(order="F")
         ad = float(np.dot(g, bv))
         eps = min(max(args.fd_target * J / abs(ad), 1e-3), 5e-2) if ad else 1e-2
         Js = {}
@@ -304,7 +382,7 @@ def fd_check_3d(args, pa, m, g, J, dobs, loss, mp, start, G):
             pa.update()
             Js[k] = database.lossvalue(loss, dobs, pa.data)
         fd = (8.0 * (Js[1] - Js[-1]) - (Js[2] - Js[-2])) / (12.0 * eps)
-        out.append({"cell": [int(x) for x in c], "eps": eps, "fd": fd, "adjoint": ad,
+        out.append({"cell": [int(x) for x in c], "param": mp[blk], "eps": eps, "fd": fd, "adjoint": ad,
                     "rel_err": abs(ad / fd - 1) if fd else None})
         print(json.dumps({"fd": out[-1]}), flush=True)
     pa.update_model(m, mp)
@@ -320,7 +398,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt", type=int, default=0, help="time steps (default: the workload's own)")
-    ap.add_argument("--workload", choices=["elastic2d", "acoustic3d", "gradient3d", "serve"], default="elastic2d")
+    ap.add_argument("--workload", choices=["elastic2d", "acoustic3d", "gradient3d", "gradient3d_elastic", "serve"],
+                    default="elastic2d")
     ap.add_argument("--checkpoint", type=int, default=100, help="gradient3d: checkpoint interval")
     ap.add_argument("--fd-cells", type=int, default=0, help="gradient3d: finite-difference checks")
     ap.add_argument("--fd-target", type=float, default=1e-3, help="gradient3d: relative misfit change per FD step")
@@ -361,6 +440,9 @@ def main():
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": sec["workload"], "max_shots": args.max_shots, "chunk_steps": args.chunk_steps},
             "serving": sec}), flush=True)
+        return
+    if args.workload == "gradient3d_elastic":
+        print(json.dumps(run_gradient3d_elastic(args, G, _lib, gallery)), flush=True)
         return
     if args.workload == "gradient3d":
         print(json.dumps(run_gradient3d(args, G, _lib, gallery)), flush=True)

@@ -182,6 +182,48 @@ def acoustic_fwi_3d(n: int = 400, ds: float = 10.0, nt: int = 3000, npml: int = 
     return true, start, ageom, srcwav, tgrid, kw
 
 
+def elastic_fwi_3d(n: int = 256, ds: float = 5.0, nt: int = 2000, npml: int = 20, freq: float = 25.0,
+                   nrec_side: int = 64, seed: int = 5005):
+    """A 3-D elastic FWI gradient problem: three layers (vp 1800 / 2600 / 3500 m/s, vs =
+    vp/1.9 / vp/1.8 / vp/1.75, Gardner density 310 vp^0.25), the deeper interface dipping
+    along x at a seeded angle in [10, 30] degrees; an explosive (tauxx) 25 Hz Ricker five
+    cells below the surface at the centre; an nrec_side x nrec_side receiver patch on the
+    surface spanning the model, recording vx, vy and vz; dt at CFL 0.5 of sqrt(vp^2+vs^2).
+    The starting model is the same medium with the dipping interface one cell shallower.
+
+    Returns (true medium, starting medium, ageom, srcwav, tgrid, kw)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    L = (n - 1) * ds
+    grid = [Grid.range(0.0, L, length=n) for _ in range(3)]
+    theta = np.deg2rad(rng.uniform(10.0, 30.0))
+    z1 = rng.uniform(0.25, 0.35) * L
+    z2c = rng.uniform(0.55, 0.65) * L
+    z = grid[0].values()[:, None, None]
+    x = grid[2].values()[None, None, :]
+
+    def medium(shift):
+        z2 = z2c + np.tan(theta) * (x - 0.5 * L) - shift * ds
+        layer = (z >= z1).astype(np.int32) + (z >= z2).astype(np.int32)   # 0, 1, 2
+        layer = np.broadcast_to(layer, (n, n, n))
+        vp = np.choose(layer, [1800.0, 2600.0, 3500.0])
+        vs = vp / np.choose(layer, [1.9, 1.8, 1.75])
+        rho = 310.0 * vp ** 0.25
+        f = lambda a: np.asfortranarray(a, dtype=np.float32)
+        return ElasticMedium(grid, f(vp), f(vs), f(rho))
+
+    true, start = medium(0.0), medium(1.0)
+    dt = 0.5 * ds / float(np.sqrt(3500.0 ** 2 + (3500.0 / 1.75) ** 2))
+    tgrid = Grid(0.0, dt, nt)
+    rp = np.linspace(0.0, L, nrec_side)
+    ry, rx = np.meshgrid(rp, rp, indexing="ij")
+    ageom = [AGeomss({"z": [5 * ds], "y": [0.5 * L], "x": [0.5 * L]},
+                     {"z": np.zeros(rx.size), "y": ry.ravel(), "x": rx.ravel()})]
+    src = Srcs(1, tgrid, ["tauxx"])
+    src.update(["tauxx"], _long_ricker(freq, tgrid) * 1e6)
+    kw = dict(rfields=["vx", "vy", "vz"], npml=npml)
+    return true, start, ageom, [src], tgrid, kw
+
+
 def _long_ricker(freq: float, tgrid: Grid) -> np.ndarray:
     """Ricker on tgrid, computed on a grid long enough for its delay and then cut."""
     need = int(math.ceil(3.0 / freq / tgrid.step)) + 2
