@@ -339,6 +339,102 @@ def run_gradient3d_elastic(args, G, _lib, gallery):
     return line
 
 
+def run_serve3d(args, G, _lib, gallery):
+    """3-D elastic shots, a fraction of them gradient requests, through the job queue
+    (worker streams, HBM admission) against the plain path (one call after another).
+    Every shot is a gallery.elastic_fwi_3d problem of seeded size and length: forward
+    shots model the true medium, gradient shots the start medium against the true one's
+    records (made before the timed region).  Both paths start from prepared experiments;
+    coefficient derivation and upload are inside the timed span of both."""
+    import numpy as np
+    from geophyinv.serve import JobServer
+
+    rng = np.random.Generator(np.random.PCG64(5005))
+    ns = args.serve_shots if args.serve_shots != 512 else 16
+    side = args.serve_side if args.serve_side != [256, 2048] else [96, 160]
+    steps = args.serve_steps if args.serve_steps != [1000, 6000] else [600, 1200]
+    ctx = _lib.Context(0)
+    shots = []
+    t0 = time.perf_counter()
+    for k in range(ns):
+        n = int(rng.integers(side[0], side[1] + 1))
+        nt = int(rng.integers(steps[0], steps[1] + 1))
+        grad = bool(rng.random() < args.grad_fraction)
+        true, start, ageom, srcwav, tgrid, kw = gallery.elastic_fwi_3d(n=n, nt=nt, seed=5005 + k)
+        if grad:
+            pt = G.SeisForwExpt(G.FdtdElastic(), medium=true, tgrid=tgrid, ageom=ageom, srcwav=srcwav, context=ctx,
+                                **kw)
+            pt.update()
+            dobs = [d.copy() for d in pt.data]
+            pt.free()
+            mk = lambda start=start, tgrid=tgrid, ageom=ageom, srcwav=srcwav, kw=kw: G.SeisForwExpt(
+                G.FdtdElastic("forward_save", 2), medium=start, tgrid=tgrid, ageom=ageom, srcwav=srcwav,
+                context=ctx, checkpoint=args.checkpoint, **kw)
+        else:
+            dobs = None
+            mk = lambda true=true, tgrid=tgrid, ageom=ageom, srcwav=srcwav, kw=kw: G.SeisForwExpt(
+                G.FdtdElastic(), medium=true, tgrid=tgrid, ageom=ageom, srcwav=srcwav, context=ctx, **kw)
+        shots.append(dict(n=n + 2 * kw["npml"], nt=nt, grad=grad, dobs=dobs, mk=mk))
+    gen_s = time.perf_counter() - t0
+    loss = G.L2DistLoss()
+    # plain path
+    pas = [sh["mk"]() for sh in shots]
+    lat_p, out_p = [], []
+    t0 = time.perf_counter()
+    for sh, pa in zip(shots, pas):
+        if sh["grad"]:
+            m = pa.model_vector()
+            G.gradient(np.zeros_like(m), m, loss, sh["dobs"], pa)
+            out_p.append({k: v.copy() for k, v in pa.gradients.items()})
+        else:
+            out_p.append(pa.run_shot(0)[0])
+        pa.free()
+        lat_p.append(time.perf_counter() - t0)
+    span_p = time.perf_counter() - t0
+    # job queue: every shot submitted at t = 0 (from the same model state as the plain run)
+    pas = [sh["mk"]() for sh in shots]
+    for sh, pa in zip(shots, pas):
+        if sh["grad"]:
+            pa.update_model(pa.model_vector())   # the model round trip the plain gradient made
+    with JobServer(workers=args.workers) as js:
+        t0 = time.perf_counter()
+        tickets = [js.submit(pa, 0, dobs=sh["dobs"][0] if sh["grad"] else None) for sh, pa in zip(shots, pas)]
+        out_s = []
+        for t in tickets:
+            out_s.append(js.wait(t))
+        span_s = time.perf_counter() - t0
+        st = js.stats()
+    same = 0
+    for sh, a, b in zip(shots, out_p, out_s):
+        if sh["grad"]:
+            same += all(np.array_equal(a[k], b["gradients"][k]) for k in b["gradients"])
+        else:
+            same += all(np.array_equal(a[f], b["records"][f]) for f in a)
+    cells = float(sum(sh["n"] ** 3 * sh["nt"] * (3 if sh["grad"] else 1) for sh in shots))
+    pct = lambda a, q: round(float(np.percentile(np.asarray(a) * 1e3, q)), 1)
+    ctx.close()
+    return {
+        "metric": "shots/s (3-D elastic job queue, forward and gradient requests)",
+        "value": round(ns / span_s, 3), "unit": "shots/s", "n_gpus": 1, "steps": ns, "warmup": 0,
+        "ms_per_step": round(span_s / ns * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{ns} 3-D elastic shots of gallery.elastic_fwi_3d, side U{{{side[0]}..{side[1]}}} "
+                               f"+ 40 CPML cells, nt U{{{steps[0]}..{steps[1]}}}, {args.grad_fraction:.0%} "
+                               f"gradient requests (checkpoints every {args.checkpoint}), all queued at t = 0",
+                   "workers": args.workers},
+        "gradient_shots": int(sum(sh["grad"] for sh in shots)),
+        "plain": {"shots_per_s": round(ns / span_p, 3), "span_s": round(span_p, 2), "p50_ms": pct(lat_p, 50),
+                  "p99_ms": pct(lat_p, 99)},
+        "queued": {"shots_per_s": round(ns / span_s, 3), "span_s": round(span_s, 2), "p50_ms": round(st["p50_ms"], 1),
+                   "p99_ms": round(st["p99_ms"], 1), "forward_p50_ms": round(st["forward_p50_ms"], 1),
+                   "gradient_p50_ms": round(st["gradient_p50_ms"], 1)},
+        "speedup_shots_per_s": round(span_p / span_s, 3),
+        "results_identical": f"{same}/{ns}",
+        "gpt_updates_per_s": round(cells / span_s / 1e9, 2),
+        "generation_s": round(gen_s, 1),
+    }
+
+
 def state_floats_3d_elastic(shape, npml):
     """Floats of one 3-D elastic state: nine wavefield slabs and the CPML memory of the
     twelve derivative fields (two strips of npml per derivative along its own axis)."""
@@ -398,8 +494,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt", type=int, default=0, help="time steps (default: the workload's own)")
-    ap.add_argument("--workload", choices=["elastic2d", "acoustic3d", "gradient3d", "gradient3d_elastic", "serve"],
-                    default="elastic2d")
+    ap.add_argument("--workload", choices=["elastic2d", "acoustic3d", "gradient3d", "gradient3d_elastic", "serve",
+                                           "serve3d"], default="elastic2d")
+    ap.add_argument("--grad-fraction", type=float, default=0.25, help="serve3d: share of gradient requests")
+    ap.add_argument("--workers", type=int, default=2, help="serve3d: jobs in flight")
     ap.add_argument("--checkpoint", type=int, default=100, help="gradient3d: checkpoint interval")
     ap.add_argument("--fd-cells", type=int, default=0, help="gradient3d: finite-difference checks")
     ap.add_argument("--fd-target", type=float, default=1e-3, help="gradient3d: relative misfit change per FD step")
@@ -440,6 +538,9 @@ def main():
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": sec["workload"], "max_shots": args.max_shots, "chunk_steps": args.chunk_steps},
             "serving": sec}), flush=True)
+        return
+    if args.workload == "serve3d":
+        print(json.dumps(run_serve3d(args, G, _lib, gallery)), flush=True)
         return
     if args.workload == "gradient3d_elastic":
         print(json.dumps(run_gradient3d_elastic(args, G, _lib, gallery)), flush=True)

@@ -90,6 +90,24 @@ class gpi_server_stats(C.Structure):
                 ("pool_bytes", C.c_uint64)]
 
 
+class gpi_jobs_opts(C.Structure):
+    _fields_ = [("workers", C.c_int32), ("reserved", C.c_int32), ("hbm_budget_bytes", C.c_uint64)]
+
+
+class gpi_job(C.Structure):
+    _fields_ = [("grid", gpi_grid), ("coefs", C.POINTER(gpi_array)), ("ncoefs", C.c_int32),
+                ("npml_arrays", C.c_int32), ("pml", C.POINTER(gpi_pml)), ("shot", gpi_shot),
+                ("checkpoint", C.c_int32), ("dtI", C.c_float), ("dobs", C.POINTER(C.POINTER(C.c_float))),
+                ("misfit", C.POINTER(C.c_double)), ("grad", C.POINTER(C.c_double) * 3)]
+
+
+class gpi_jobs_stats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64), ("queued", C.c_int64),
+                ("running", C.c_int64), ("forward_done", C.c_int64), ("gradient_done", C.c_int64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("forward_p50_ms", C.c_double),
+                ("gradient_p50_ms", C.c_double), ("hbm_used", C.c_uint64), ("hbm_budget", C.c_uint64)]
+
+
 class gpi_timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("velocity_avg_ms", C.c_double), ("stress_avg_ms", C.c_double),
                 ("steps", C.c_int64), ("sampled", C.c_int64), ("launches", C.c_int64)]
@@ -117,6 +135,12 @@ SIGNATURES = {
     "gpi_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32)]),
     "gpi_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(C.c_int32)]),
     "gpi_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(gpi_server_stats)]),
+    "gpi_jobs_create": (C.c_int, [C.c_int, C.POINTER(gpi_jobs_opts), C.POINTER(C.c_void_p)]),
+    "gpi_jobs_destroy": (None, [C.c_void_p]),
+    "gpi_jobs_error": (C.c_char_p, [C.c_void_p]),
+    "gpi_jobs_submit": (C.c_int, [C.c_void_p, C.POINTER(gpi_job), C.POINTER(C.c_uint64)]),
+    "gpi_jobs_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(C.c_int32)]),
+    "gpi_jobs_get_stats": (C.c_int, [C.c_void_p, C.POINTER(gpi_jobs_stats)]),
     "gpi_latency": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double)]),
     "gpi_forward": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gpi_shot)]),
     "gpi_forward_save": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gpi_shot), C.POINTER(C.c_uint64)]),

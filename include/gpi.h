@@ -360,6 +360,57 @@ int gpi_server_get_stats(gpi_server* srv, gpi_server_stats* out);
  * finished shot (GPI_ENOTFOUND until then). */
 int gpi_latency(gpi_server* srv, uint64_t ticket, double* ms);
 
+/* ---- job queue: mixed forward and gradient jobs, 2-D or 3-D -------------------------- */
+/* No reference counterpart.  Jobs wait in arrival order and start when a worker is free
+ * and their estimated device memory fits the HBM budget; each worker owns a context and
+ * stream, so the device work, the host work (coefficient upload, residuals) and the
+ * copies of several jobs overlap.  A forward job is gpi_forward; a gradient job is
+ * gpi_forward_checkpoint, the misfit sum (dcal - dobs)^2 in FP64 and the adjoint sources
+ * 2 (dobs - dcal) in FP32 reversed in time (database.jl:357-380), then
+ * gpi_adjoint_checkpoint (acoustic) or gpi_adjoint_checkpoint_elastic; its results equal
+ * those calls made one after another (GPU test).  Every buffer a job names (coefficients,
+ * CPML vectors, points, wavelets, observed records and the outputs) must stay valid until
+ * the job is done. */
+typedef struct gpi_jobs gpi_jobs;
+
+typedef struct gpi_jobs_opts {
+    int32_t workers;             /* jobs in flight at most (default 2) */
+    int32_t reserved;            /* 0 */
+    uint64_t hbm_budget_bytes;   /* 0: 90 % of the free HBM at creation */
+} gpi_jobs_opts;
+
+typedef struct gpi_job {
+    gpi_grid grid;
+    const gpi_array* coefs;      /* as gpi_model_create */
+    int32_t ncoefs;
+    int32_t npml_arrays;
+    const gpi_pml* pml;
+    gpi_shot shot;               /* records into shot.records; nfinal = nsnap = 0 */
+    int32_t checkpoint;          /* 0: forward job; > 0: gradient job, checkpoint interval */
+    float dtI;                   /* gradient jobs, acoustic: Float32(1/dt) of the time grid */
+    const float* const* dobs;    /* gradient jobs: observed records per receiver set, nt x npts */
+    double* misfit;              /* gradient jobs: output */
+    double* grad[3];             /* gradient jobs: stress-grid FP64 outputs, invK and rho (grad[1]
+                                    may be NULL) or invlambda, invmu and rho */
+} gpi_job;
+
+typedef struct gpi_jobs_stats {
+    int64_t submitted, completed, failed, queued, running;
+    int64_t forward_done, gradient_done;
+    double p50_ms, p99_ms;              /* submission to outputs written, all jobs */
+    double forward_p50_ms, gradient_p50_ms;
+    uint64_t hbm_used, hbm_budget;      /* estimated bytes of the running jobs */
+} gpi_jobs_stats;
+
+int gpi_jobs_create(int device, const gpi_jobs_opts* opts, gpi_jobs** out);
+/* Waits for every submitted job, then stops the workers. */
+void gpi_jobs_destroy(gpi_jobs* q);
+const char* gpi_jobs_error(const gpi_jobs* q);
+int gpi_jobs_submit(gpi_jobs* q, const gpi_job* job, uint64_t* ticket);
+/* status: 0 done, 1 pending, < 0 the job's error code (message in gpi_jobs_error). */
+int gpi_jobs_wait(gpi_jobs* q, uint64_t ticket, double timeout_s, int32_t* status);
+int gpi_jobs_get_stats(gpi_jobs* q, gpi_jobs_stats* out);
+
 /* Device timing of the last gpi_forward on this context. */
 int gpi_last_timing(const gpi_ctx* ctx, gpi_timing* out);
 
