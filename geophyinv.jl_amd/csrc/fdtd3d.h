@@ -25,6 +25,7 @@ struct VelParams3D {
     int gx, gy;        // workgroups along z and y; the grid's third factor runs over x
     int kc;            // x planes per workgroup (acoustic passes march along x)
     int remap;         // XCD-aware workgroup order
+    int xfast;         // elastic: consecutive workgroups walk along x (then z, y tiles)
     uint32_t pml_mask, rigid_mask;
     float dxI, dyI, dzI, dt;
 };
@@ -47,6 +48,7 @@ struct StrParams3D {
     int gx, gy;
     int kc;            // x planes per workgroup (acoustic passes march along x)
     int remap;         // XCD-aware workgroup order
+    int xfast;         // elastic: consecutive workgroups walk along x (then z, y tiles)
     uint32_t pml_mask, free_mask;
     float dxI, dyI, dzI;
 };
