@@ -37,6 +37,16 @@ def algorithmic_bytes_3d(n, npml):
     return 40 * N + 8 * strips, 24 * N + 8 * strips
 
 
+def algorithmic_bytes_3d_elastic(n, npml):
+    """3-D elastic: velocity pass reads the six stresses, vx, vy, vz and rho and writes the
+    velocities (52 B per cell); stress pass reads vx, vy, vz, the six stresses, invlambda and
+    invmu and writes the stresses (68 B per cell); three CPML memory variables read and
+    written per strip cell per axis in each pass (24 B)."""
+    N = n[0] * n[1] * n[2]
+    strips = sum(strip_cells(N, n[a], npml, 2) for a in range(3))
+    return 52 * N + 24 * strips, 68 * N + 24 * strips
+
+
 def algorithmic_bytes(nz, nx, npml, physics):
     """Compulsory FP32 bytes of one velocity pass and one stress pass (each array read once,
     each output written once; derivatives never stored; CPML memory read+written in strips;
@@ -494,8 +504,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt", type=int, default=0, help="time steps (default: the workload's own)")
-    ap.add_argument("--workload", choices=["elastic2d", "acoustic3d", "gradient3d", "gradient3d_elastic", "serve",
-                                           "serve3d"], default="elastic2d")
+    ap.add_argument("--workload", choices=["elastic2d", "acoustic3d", "elastic3d", "gradient3d", "gradient3d_elastic",
+                                           "serve", "serve3d"], default="elastic2d")
     ap.add_argument("--grad-fraction", type=float, default=0.25, help="serve3d: share of gradient requests")
     ap.add_argument("--workers", type=int, default=2, help="serve3d: jobs in flight")
     ap.add_argument("--checkpoint", type=int, default=100, help="gradient3d: checkpoint interval")
@@ -548,8 +558,13 @@ def main():
     if args.workload == "gradient3d":
         print(json.dumps(run_gradient3d(args, G, _lib, gallery)), flush=True)
         return
-    three = args.workload == "acoustic3d"
-    if three:
+    three = args.workload in ("acoustic3d", "elastic3d")
+    el3 = args.workload == "elastic3d"
+    if el3:
+        med, _, ageom, srcwav, tgrid, kw = gallery.elastic_fwi_3d(n=args.grid_n if args.grid_n != 400 else 320,
+                                                                  nt=args.nt or 1000)
+        attrib = G.FdtdElastic()
+    elif three:
         med, ageom, srcwav, tgrid, kw = gallery.acoustic_gradient_3d(nt=args.nt or 3000)
         attrib = G.FdtdAcoustic()
     else:
@@ -562,7 +577,7 @@ def main():
     cells = int(np.prod(shape))
     nt = len(tgrid)
     # the profiled workloads: their measured traffic per launch is in profiles/
-    profiled = pa.order == 2 and shape in ((2081, 2081), (440, 440, 440))
+    profiled = pa.order == 2 and shape in ((2081, 2081), (440, 440, 440), (360, 360, 360))
     for _ in range(args.warmup):
         pa.run_shot(0)
     dev_ms, vel_ms, str_ms = [], [], []
@@ -577,7 +592,9 @@ def main():
     finite = all(np.all(np.isfinite(r)) for r in recs.values())
     gpt = cells * nt * args.steps
     value = gpt / wall / 1e9
-    if three:
+    if el3:
+        vel_b, str_b = algorithmic_bytes_3d_elastic(shape, pa.npml)
+    elif three:
         vel_b, str_b = algorithmic_bytes_3d(shape, pa.npml)
     else:
         vel_b, str_b = algorithmic_bytes(shape[0], shape[1], pa.npml, 1)
@@ -590,22 +607,32 @@ def main():
     step_s = float(np.mean(dev_ms)) * 1e-3 / nt
     s_avg = step_s * s_br / (s_br + v_br)
     v_avg = step_s * v_br / (s_br + v_br)
-    # the dominant kernel: the 2-D elastic stress pass; in 3-D the velocity pass
-    dom_b, dom_t = (vel_b, v_avg) if three else (str_b, s_avg)
-    other_b, other_t, other_br = (str_b, s_avg, s_br) if three else (vel_b, v_avg, v_br)
+    # the dominant kernel: the elastic stress passes; the 3-D acoustic velocity pass
+    vdom = three and not el3
+    dom_b, dom_t = (vel_b, v_avg) if vdom else (str_b, s_avg)
+    other_b, other_t, other_br = (str_b, s_avg, s_br) if vdom else (vel_b, v_avg, v_br)
     achieved = dom_b / dom_t / 1e9
     step_bytes = vel_b + str_b
-    if three:
+    if el3:
+        metric = "Gpt-updates/s (3-D elastic FDTD, FP32)"
+        workload = (f"3-D elastic FdtdElastic, {shape[0] - 40}^3 at 5 m, 20-cell CPML ({shape[0]}^3), three layers "
+                    "vp 1800/2600/3500 m/s with a seeded dipping interface, explosive 25 Hz Ricker, 64x64 surface "
+                    "receivers (vx, vy, vz), one shot per step")
+        kname, other_name = "str3d_elastic<2> (stress pass)", "velocity_pass"
+        trf = measured_traffic("fdtd3d_elastic_360_traffic.md", "str3d_elastic")
+    elif three:
         metric = "Gpt-updates/s (3-D acoustic FDTD, FP32)"
         workload = ("3-D acoustic FdtdAcoustic, 400^3 at 10 m, 20-cell CPML (440^3), vp 1500->4000 m/s "
                     "gradient + three +-8% spheres, 12 Hz Ricker on p, 201x201 surface receivers (p), "
                     "one shot per step")
         kname, other_name = "vel3d_acoustic<2> (velocity pass)", "stress_pass"
+        trf = measured_traffic("fdtd3d_acoustic_440_traffic.md", "vel3d_acoustic")
     else:
         metric = "Gpt-updates/s (2-D elastic FDTD, FP32)"
         workload = ("2-D elastic P-SV FdtdElastic, 2001x2001 at 2.5 m, 40-cell CPML (2081x2081), 8 layers + "
                     "3% smoothed noise, explosive 20 Hz Ricker, 801 receivers (vx, vz), one shot per step")
         kname, other_name = "str2d_elastic<2> (stress pass)", "velocity_pass"
+        trf = measured_traffic("fdtd2d_elastic_2081x2081_traffic.md", "str2d_elastic")
     line = {
         "metric": metric,
         "value": round(value, 3),
@@ -626,11 +653,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": (measured_traffic("fdtd3d_acoustic_440_traffic.md", "vel3d_acoustic") if three
-                                 else measured_traffic("fdtd2d_elastic_2081x2081_traffic.md", "str2d_elastic"))
-                                if profiled else None,
+                     "traffic": trf if profiled else None,
                      "bytes_per_launch": int(dom_b), "avg_launch_ms": round(dom_t * 1e3, 5),
-                     "bracket_ms": round((v_br if three else s_br) * 1e3, 5),
+                     "bracket_ms": round((v_br if vdom else s_br) * 1e3, 5),
                      other_name: {"bytes_per_launch": int(other_b), "avg_launch_ms": round(other_t * 1e3, 5),
                                   "bracket_ms": round(other_br * 1e3, 5),
                                   "achieved": round(other_b / other_t / 1e9, 1)}},
