@@ -53,12 +53,12 @@ struct StrParams3D {
     float dxI, dyI, dzI;
 };
 
-// Rows of the 3-D acoustic velocity pass are shifted by h: workgroup row bx covers
-// derivative-grid rows 64*bx - h .. 64*bx + 63 - h, so the velocity and pressure columns
-// it reads and writes at row i + h start on a 128-byte boundary (no cache line is shared
-// with the neighbouring workgroup, which may run on another XCD).  The density
-// coefficients it reads at row i are stored h floats past their allocation for the same
-// reason.  Returns the shift (0 for elastic).
+// Rows of the 3-D velocity passes are shifted by h: workgroup row bx covers
+// derivative-grid rows 64*bx - h .. 64*bx + 63 - h, so the velocity (and, acoustic,
+// pressure) columns it reads and writes at row i + h start on a 128-byte boundary (no
+// cache line is shared with the neighbouring workgroup, which may run on another XCD).
+// The acoustic density coefficients it reads at row i are stored h floats past their
+// allocation for the same reason.  Returns the shift.
 int vel3d_row_shift(const Model& m);
 // y rows per workgroup of the 3-D passes: acoustic GPI_3D_TY (4 or 8, default 4: on 440^3
 // the 64 x 8 tile ran 9 % slower), elastic 4

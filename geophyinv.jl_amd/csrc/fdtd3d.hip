@@ -285,14 +285,15 @@ __global__ __launch_bounds__(256) void vel3d_elastic(VelParams3D P, PassExtras X
     }
     Idx3 t;
     const int bt = decode(b, P.gx, P.gy, nxw, P.xfast != 0, t);
-    const int i = t.i, j = t.j, k = t.k;
+    const int i = t.i - H, j = t.j, k = t.k;   // rows shifted by h: the velocity rows start on 128-byte lines
     const int s1 = (int)P.s1, s2 = (int)P.s2;
     const float *txx = P.s[0], *tyy = P.s[1], *tzz = P.s[2], *txy = P.s[3], *txz = P.s[4], *tyz = P.s[5];
     const int o = (i + H) + s1 * (j + H) + s2 * (k + H);
     const int c = i + s1 * j + s2 * k;
-    const bool ux = i < P.nz - 2 * H && j < P.ny - 2 * H && k < P.nx - H;
-    const bool uy = i < P.nz - 2 * H && j < P.ny - H && k < P.nx - 2 * H;
-    const bool uz = i < P.nz - H && j < P.ny - 2 * H && k < P.nx - 2 * H;
+    const bool row = i >= 0;
+    const bool ux = row && i < P.nz - 2 * H && j < P.ny - 2 * H && k < P.nx - H;
+    const bool uy = row && i < P.nz - 2 * H && j < P.ny - H && k < P.nx - 2 * H;
+    const bool uz = row && i < P.nz - H && j < P.ny - 2 * H && k < P.nx - 2 * H;
     // loads of the three updates first, then evaluation and stores
     float a1[O], a2[O], a3[O], b1[O], b2[O], b3[O], e1[O], e2[O], e3[O];
     float vxo = 0.0f, vyo = 0.0f, vzo = 0.0f;
@@ -568,7 +569,7 @@ void launch_str3(const Model& m, hipStream_t s, StrParams3D P, const PassExtras&
 
 }  // namespace
 
-int vel3d_row_shift(const Model& m) { return m.g.physics == 0 ? m.h : 0; }
+int vel3d_row_shift(const Model& m) { return m.h; }
 
 int tile_rows_3d(const Model& m)
 {
