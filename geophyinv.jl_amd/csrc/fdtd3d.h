@@ -61,7 +61,8 @@ struct StrParams3D {
 // allocation for the same reason.  Returns the shift.
 int vel3d_row_shift(const Model& m);
 // y rows per workgroup of the 3-D passes: acoustic GPI_3D_TY (4 or 8, default 4: on 440^3
-// the 64 x 8 tile ran 9 % slower), elastic 4
+// the 64 x 8 tile ran 9 % slower), elastic GPI_3D_TYE (4 or 8, default 4: on 360^3 the
+// 64 x 8 tile ran 12 % slower)
 int tile_rows_3d(const Model& m);
 int velocity_blocks_3d(const Model& m, int* gx, int* gy);
 int stress_blocks_3d(const Model& m, int* gx, int* gy);

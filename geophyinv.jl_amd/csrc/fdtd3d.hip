@@ -97,7 +97,7 @@ __device__ __forceinline__ int decode(int b, int gx, int gy, int nk, bool xfast,
         p.k = r / gy;
     }
     p.i = bx * 64 + (int)threadIdx.x;
-    p.j = by * 4 + (int)threadIdx.y;
+    p.j = by * (int)blockDim.y + (int)threadIdx.y;
     return bx + gx * (by + gy * p.k);
 }
 
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(512) void str3d_acoustic(StrParams3D P, PassExtras 
 // ---------------------------------------------------------------------------------------
 
 template <int O>
-__global__ __launch_bounds__(256) void vel3d_elastic(VelParams3D P, PassExtras X)
+__global__ __launch_bounds__(512) void vel3d_elastic(VelParams3D P, PassExtras X)
 {
     constexpr int H = O - 1;
     const int b = P.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256) void vel3d_elastic(VelParams3D P, PassExtras X
 }
 
 template <int O>
-__global__ __launch_bounds__(256) void str3d_elastic(StrParams3D P, PassExtras X)
+__global__ __launch_bounds__(512) void str3d_elastic(StrParams3D P, PassExtras X)
 {
     constexpr int H = O - 1;
     const int b = P.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
@@ -578,7 +578,12 @@ int tile_rows_3d(const Model& m)
         const char* e = getenv("GPI_3D_TY");
         ty = e && atoi(e) == 8 ? 8 : 4;
     }
-    return m.g.physics == 0 ? ty : 4;
+    static int tye = -1;
+    if (tye < 0) {
+        const char* e = getenv("GPI_3D_TYE");
+        tye = e && atoi(e) == 8 ? 8 : 4;
+    }
+    return m.g.physics == 0 ? ty : tye;
 }
 
 int velocity_blocks_3d(const Model& m, int* gx, int* gy)
