@@ -360,9 +360,9 @@ def run_serve3d(args, G, _lib, gallery):
     from geophyinv.serve import JobServer
 
     rng = np.random.Generator(np.random.PCG64(5005))
-    ns = args.serve_shots if args.serve_shots != 512 else 16
-    side = args.serve_side if args.serve_side != [256, 2048] else [96, 160]
-    steps = args.serve_steps if args.serve_steps != [1000, 6000] else [600, 1200]
+    ns = args.serve_shots or 16
+    side = args.serve_side or [96, 160]
+    steps = args.serve_steps or [600, 1200]
     ctx = _lib.Context(0)
     shots = []
     t0 = time.perf_counter()
@@ -514,18 +514,20 @@ def main():
     ap.add_argument("--grid-n", type=int, default=400, help="gradient3d: cells per side of the medium")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-shots", type=int, default=512)
-    # The serving-stream measurement is opt-in: its full-size run faulted the GPU once and
-    # the cause has not been shown, so the default run leaves it out.
-    ap.add_argument("--serve", action="store_true")
+    # Serving stream (the default 2-D elastic run appends it; --no-serve leaves it out).  The
+    # defaults are 128 shots of at most 512 cells per side, all queued at once, prepared on
+    # the host before the stream: a 48-shot sample of the full 256..2048-cell stream faulted
+    # the GPU once and the cause has not been shown, so sizes beyond 512 need the flags.
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-shots", type=int, default=None, help="serving: shots (128; serve3d 16)")
     ap.add_argument("--max-shots", type=int, default=64)
     ap.add_argument("--chunk-steps", type=int, default=32)
-    ap.add_argument("--serve-side", type=int, nargs=2, default=[256, 2048], help="serving: grid side range")
-    ap.add_argument("--serve-steps", type=int, nargs=2, default=[1000, 6000], help="serving: time step range")
+    ap.add_argument("--serve-side", type=int, nargs=2, default=None, help="serving: grid side range (128 512)")
+    ap.add_argument("--serve-steps", type=int, nargs=2, default=None, help="serving: time step range (500 1500)")
     ap.add_argument("--serve-direct", action="store_true", help="serving: also run without graph replay")
-    ap.add_argument("--serve-rate", type=float, default=200.0, help="serving: Poisson arrival rate (1/s)")
-    ap.add_argument("--serve-prebuilt", action="store_true",
-                    help="serving: prepare every shot on the host before the stream starts (both paths)")
+    ap.add_argument("--serve-rate", type=float, default=1e6, help="serving: Poisson arrival rate (1/s)")
+    ap.add_argument("--serve-host-prep", action="store_true",
+                    help="serving: prepare each shot on the host at its arrival, inside the timed span")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the single-GPU path only "
@@ -539,11 +541,12 @@ def main():
 
     if args.workload == "serve":
         opts = dict(max_shots=args.max_shots, chunk_steps=args.chunk_steps, autotune=True)
-        sec = serving_section(args.serve_shots, opts, G, _lib, args.serve_side, args.serve_steps, args.serve_direct,
-                              args.serve_rate, args.serve_prebuilt)
+        sec = serving_section(args.serve_shots or 128, opts, G, _lib, args.serve_side or [128, 512],
+                              args.serve_steps or [500, 1500], args.serve_direct, args.serve_rate,
+                              not args.serve_host_prep)
         print(json.dumps({
             "metric": "shots/s (2-D acoustic serving stream, continuous batching)",
-            "value": sec["served"]["shots_per_s"], "unit": "shots/s", "n_gpus": 1, "steps": args.serve_shots,
+            "value": sec["served"]["shots_per_s"], "unit": "shots/s", "n_gpus": 1, "steps": sec["shots"],
             "warmup": 0, "ms_per_step": round(1e3 / max(sec["served"]["shots_per_s"], 1e-9), 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": sec["workload"], "max_shots": args.max_shots, "chunk_steps": args.chunk_steps},
@@ -664,9 +667,11 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(pa, args.cpu_seconds)
     ctx.close()
-    if args.serve:
+    if not args.no_serve and not three:
         opts = dict(max_shots=args.max_shots, chunk_steps=args.chunk_steps, autotune=True)
-        line["serving"] = serving_section(args.serve_shots, opts, G, _lib)
+        line["serving"] = serving_section(args.serve_shots or 128, opts, G, _lib, args.serve_side or [128, 512],
+                                          args.serve_steps or [500, 1500], args.serve_direct, args.serve_rate,
+                                          not args.serve_host_prep)
     print(json.dumps(line), flush=True)
 
 
