@@ -743,14 +743,14 @@ int gpi_adjoint_checkpoint(gpi_ctx* c, uint64_t model, uint64_t store, const gpi
     if (rho)
         for (int d = 0; d < nv; ++d)
             if ((rc = S.alloc(&segV[d], (size_t)(L + 1) * slab)) || (rc = S.alloc(&vnext[d], slab))) return rc;
-    // adjoint fields before the step, two alternating sets (p, vx, vy, vz)
+    // adjoint velocities before the step (for rho), two alternating sets; the adjoint p
+    // before the step is the one the stress pass starts from (the velocity pass leaves p),
+    // so the imaging reads it there and keeps no copy
     float* tp[2][4] = {};
-    for (auto& set : tp) {
-        if ((rc = S.alloc(&set[0], slab))) return rc;
-        if (rho)
+    if (rho)
+        for (auto& set : tp)
             for (int d = 0; d < nv; ++d)
                 if ((rc = S.alloc(&set[1 + d], slab))) return rc;
-    }
     double *gK = nullptr, *gR = nullptr;
     if ((rc = S.alloc(&gK, slab))) return rc;
     if (rho && (rc = S.alloc(&gR, slab))) return rc;
@@ -800,28 +800,29 @@ int gpi_adjoint_checkpoint(gpi_ctx* c, uint64_t model, uint64_t store, const gpi
             if ((rc = velocity_pass(c, m, s2, XV2))) return rc;
             if (E) HIPTRY(c, hipEventRecord(E[1], c->stream));
             for (int q : src2.vfall) inject(m, s2, src2.pts[q], it, c->stream);
-            if (E) HIPTRY(c, hipEventRecord(E[2], c->stream));
-            if ((rc = stress_pass(c, m, s2, XS2))) return rc;
-            if (E) HIPTRY(c, hipEventRecord(E[3], c->stream));
             ImageSets I{};
             float* const* prev = tp[it & 1];
             float* const* next = tp[(it + 1) & 1];
-            I.a2[0] = prev[0];
-            I.c2[0] = s2.field[GPI_P];
-            I.n2[0] = next[0];
+            I.a2[0] = s2.field[GPI_P];                   // p2 before the step
             I.pa = segP + (size_t)(n - n0) * slab;       // p_{n-1}
             I.pb = segP + (size_t)(n - n0 + 1) * slab;   // p_n
-            if (rho)
+            image_acoustic_exact(m, I, gK, nullptr, dtI, c->stream);   // invK term
+            if (E) HIPTRY(c, hipEventRecord(E[2], c->stream));
+            if ((rc = stress_pass(c, m, s2, XS2))) return rc;
+            if (E) HIPTRY(c, hipEventRecord(E[3], c->stream));
+            if (rho) {   // density term: v2 before the step (a copy), then keep this step's v2
+                ImageSets Iv{};
                 for (int d = 0; d < nv; ++d) {
                     const int a = vf[d] - GPI_VX;   // 0 x, 1 y, 2 z
-                    I.a2[1 + a] = prev[1 + d];
-                    I.c2[1 + a] = s2.field[vf[d]];
-                    I.n2[1 + a] = next[1 + d];
-                    I.va[a] = segV[d] + (size_t)(n - n0) * slab;       // v_n
-                    I.vb[a] = segV[d] + (size_t)(n - n0 + 1) * slab;   // v_{n+1}
+                    Iv.a2[1 + a] = prev[1 + d];
+                    Iv.c2[1 + a] = s2.field[vf[d]];
+                    Iv.n2[1 + a] = next[1 + d];
+                    Iv.va[a] = segV[d] + (size_t)(n - n0) * slab;       // v_n
+                    Iv.vb[a] = segV[d] + (size_t)(n - n0 + 1) * slab;   // v_{n+1}
                 }
-            image_acoustic_exact(m, I, gK, gR, dtI, c->stream);
-            launches += 3 + (int64_t)src2.vfall.size();
+                image_acoustic_exact(m, Iv, nullptr, gR, dtI, c->stream);
+            }
+            launches += (rho ? 4 : 3) + (int64_t)src2.vfall.size();
             smp.done(it);
         }
         if (rho)

@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void image_exact_kernel(ImageExactParams P)
     const int64_t sy = P.ld, sx = (int64_t)P.ld * P.py;
     const int64_t o = i + sy * j + sx * k;
     const ImageSets& S = P.S;
-    if (i < P.nz && j < P.ny && k < P.nx) {
+    if (P.gK && i < P.nz && j < P.ny && k < P.nx) {
         // compute_gmodKI!: p2_prev * (p1_prev - p1) * dtI, p1 = -p_forward
         const float t = (S.a2[0][o] * (S.pa[o] - S.pb[o])) * P.dtI;
         P.gK[o] = P.gK[o] + (double)t;
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void image_exact_kernel(ImageExactParams P)
         }
         P.gR[g] = r - (double)avz;
     }
-    S.n2[0][o] = S.c2[0][o];
+    if (S.n2[0]) S.n2[0][o] = S.c2[0][o];
     if (P.gR)
         for (int a = 0; a < 3; ++a)
             if (S.n2[1 + a]) S.n2[1 + a][o] = S.c2[1 + a][o];

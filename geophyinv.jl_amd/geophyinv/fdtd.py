@@ -706,9 +706,12 @@ class SeisForwExpt:
         sw = [list(x) for x in srcwav] if per_pw else [srcwav]
         if len(sw) > self.npw or any(len(x) != len(self.srcwav) for x in sw):
             raise ValueError("srcwav must have one entry per supersource (and per wavefield)")
-        self.srcwav = [s.copy() for s in sw[0]]
+        # the experiment's own source sets (gradient!'s adjoint sources) are taken as they
+        # are; anything else is copied, as update! copies into the experiment
+        own = lambda s, mine: any(s is x for x in mine)
+        self.srcwav = [s if own(s, self.srcwav) else s.copy() for s in sw[0]]
         if len(sw) == 2:
-            self.srcwav2 = [s.copy() for s in sw[1]]
+            self.srcwav2 = [s if own(s, self.srcwav2) else s.copy() for s in sw[1]]
         if src_types is not None:
             st = list(src_types) if isinstance(src_types, (list, tuple)) else [src_types]
             self.src_types[:len(st)] = st
