@@ -67,6 +67,15 @@ __device__ __forceinline__ float dstride(const float* a, int off, int stride)
     return stencil<O>(v);
 }
 
+// Store of a pass output: streaming (non-temporal) when NT, for the 3-D acoustic passes
+// whose outputs are not read again before they leave the caches.
+template <bool NT>
+__device__ __forceinline__ void store_out(float* p, float v)
+{
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
 template <int O>
 __device__ __forceinline__ void load_s(float (&v)[O], const float* a, int off, int stride)
 {
@@ -111,7 +120,7 @@ __device__ __forceinline__ int decode(int b, int gx, int gy, int nk, bool xfast,
 // instead of once per plane that uses it.  The injection bucket of a point is the
 // per-plane tile id bx + gx*(by + gy*k), as planned on the host.
 
-template <int O>
+template <int O, bool NT>
 __global__ __launch_bounds__(512) void vel3d_acoustic(VelParams3D P, PassExtras X)
 {
     constexpr int H = O - 1;
@@ -182,7 +191,7 @@ __global__ __launch_bounds__(512) void vel3d_acoustic(VelParams3D P, PassExtras 
             if ((P.rigid_mask & GPI_FACE(2, 1)) && k == P.nx - H - 1)
                 P.vx[(i + H) + s1 * (j + H) + s2 * (P.nx + H - 1)] = -nv;
             if (X.ninj) nv = inject_velocity(X, INJ_VX, bt, o, nv, P.rho, P.dt);
-            P.vx[o] = nv;
+            store_out<NT>(&P.vx[o], nv);
         }
         // vy on (nz-2h, ny-h, nx-2h): dpdy = D_yi(p)
         if (ycol && k < P.nx - 2 * H) {
@@ -193,7 +202,7 @@ __global__ __launch_bounds__(512) void vel3d_acoustic(VelParams3D P, PassExtras 
             if ((P.rigid_mask & GPI_FACE(1, 1)) && j == P.ny - H - 1)
                 P.vy[(i + H) + s1 * (P.ny + H - 1) + s2 * (k + H)] = -nv;
             if (X.ninj) nv = inject_velocity(X, INJ_VY, bt, o, nv, P.rho, P.dt);
-            P.vy[o] = nv;
+            store_out<NT>(&P.vy[o], nv);
         }
         // vz on (nz-h, ny-2h, nx-2h): dpdz = D_zi(p)
         if (zcol && k < P.nx - 2 * H) {
@@ -204,13 +213,13 @@ __global__ __launch_bounds__(512) void vel3d_acoustic(VelParams3D P, PassExtras 
             if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1)
                 P.vz[(P.nz + H - 1) + s1 * (j + H) + s2 * (k + H)] = -nv;
             if (X.ninj) nv = inject_velocity(X, INJ_VZ, bt, o, nv, P.rho, P.dt);
-            P.vz[o] = nv;
+            store_out<NT>(&P.vz[o], nv);
         }
         cur = nxt;
     }
 }
 
-template <int O>
+template <int O, bool NT>
 __global__ __launch_bounds__(512) void str3d_acoustic(StrParams3D P, PassExtras X)
 {
     const int b = P.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
@@ -264,7 +273,7 @@ __global__ __launch_bounds__(512) void str3d_acoustic(StrParams3D P, PassExtras 
             int tt;
             if (X.inj[q].kind == INJ_P && injection_sum(X.inj[q], bt, o, X.it, acc, tt)) pn = pn + (acc * c);
         }
-        P.s[0][o] = pn;
+        store_out<NT>(&P.s[0][o], pn);
         cur = nxt;
     }
 }
@@ -524,6 +533,17 @@ int march_planes(int n)
     return std::min(kc, std::max(n, 1));
 }
 
+// 3-D acoustic outputs stored non-temporally when GPI_3D_NT=1.
+bool nt_stores()
+{
+    static int e = -1;
+    if (e < 0) {
+        const char* v = getenv("GPI_3D_NT");
+        e = v ? atoi(v) != 0 : 0;
+    }
+    return e == 1;
+}
+
 // Elastic workgroup order: GPI_3D_EORDER=1 walks each tile's x planes consecutively.
 // Measured on 360^3 (order 2): plain order 24.9 Gpt/s, x-fastest 23.3, x-fastest with
 // the XCD-aware order 21.2; marching along x with the x-stencils in registers (as the
@@ -549,7 +569,8 @@ void launch_vel3(const Model& m, hipStream_t s, VelParams3D P, const PassExtras&
     P.remap = remap3();
     P.xfast = m.g.physics == 1 && elastic_xfast();
     dim3 grd(P.gx * P.gy * ((P.nx - H + P.kc - 1) / P.kc) + X.gather_blocks), blk(64, ty);
-    if (m.g.physics == 0) hipLaunchKernelGGL(vel3d_acoustic<O>, grd, blk, 0, s, P, X);
+    if (m.g.physics == 0 && nt_stores()) hipLaunchKernelGGL((vel3d_acoustic<O, true>), grd, blk, 0, s, P, X);
+    else if (m.g.physics == 0) hipLaunchKernelGGL((vel3d_acoustic<O, false>), grd, blk, 0, s, P, X);
     else hipLaunchKernelGGL(vel3d_elastic<O>, grd, blk, 0, s, P, X);
 }
 
@@ -563,7 +584,8 @@ void launch_str3(const Model& m, hipStream_t s, StrParams3D P, const PassExtras&
     P.remap = remap3();
     P.xfast = m.g.physics == 1 && elastic_xfast();
     dim3 grd(P.gx * P.gy * ((P.nx + P.kc - 1) / P.kc) + X.gather_blocks), blk(64, ty);
-    if (m.g.physics == 0) hipLaunchKernelGGL(str3d_acoustic<O>, grd, blk, 0, s, P, X);
+    if (m.g.physics == 0 && nt_stores()) hipLaunchKernelGGL((str3d_acoustic<O, true>), grd, blk, 0, s, P, X);
+    else if (m.g.physics == 0) hipLaunchKernelGGL((str3d_acoustic<O, false>), grd, blk, 0, s, P, X);
     else hipLaunchKernelGGL(str3d_elastic<O>, grd, blk, 0, s, P, X);
 }
 

@@ -139,6 +139,8 @@ struct Ctx {
 // stand-alone sparse kernels (sparse.hip): injection after a pass, for source sets that
 // cannot ride along with the pass (targets on a free-surface row or dense target lists)
 void inject(const Model& m, const State& st, const DevPoints& pts, int64_t it, hipStream_t s);
+// receiver records, nt x npts point fastest -> npts x nt time fastest (the host layout)
+void records_transpose(const float* rec, float* out, int64_t nt, int npts, hipStream_t s);
 
 // time-reversal kernels (strips.hip)
 void strip_copy(const Model& m, float* field, const int64_t fn[3], int axis, int64_t np, float* store,

@@ -391,6 +391,29 @@ int finish_timing(gpi_ctx* c, const Sampler& smp, int64_t nt, int64_t launches)
     return GPI_OK;
 }
 
+// Records of every receiver set into the caller's buffers (time fastest): transposed on
+// the device into one staging buffer, then copied; the stream has drained on return.
+int copy_records(gpi_ctx* c, const std::vector<DevPoints>& rec, int64_t nt, float* const* out)
+{
+    size_t most = 0;
+    for (const DevPoints& D : rec) most = std::max(most, (size_t)nt * (size_t)std::max(D.npts, 0));
+    if (!most) return GPI_OK;
+    Scratch S(c);
+    float* tmp = nullptr;
+    int rc = S.alloc(&tmp, most);
+    if (rc) return rc;
+    for (size_t q = 0; q < rec.size(); ++q) {
+        const DevPoints& D = rec[q];
+        if (D.npts <= 0) continue;
+        records_transpose(D.rec, tmp, nt, D.npts, c->stream);
+        HIPTRY(c, hipGetLastError());
+        HIPTRY(c, hipMemcpyAsync(out[q], tmp, sizeof(float) * (size_t)nt * (size_t)D.npts, hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPTRY(c, hipStreamSynchronize(c->stream));   // tmp is reused by the next set
+    }
+    return GPI_OK;
+}
+
 // mod_x_proc! for one supersource in :forward (save == nullptr) or :forward_save mode
 // (src/fdtd/propagate.jl:138-261).
 int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
@@ -481,14 +504,7 @@ int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
     if ((rc = finish_timing(c, smp, nt, launches))) return rc;
 
     // copy back
-    for (int q = 0; q < shot->nrec; ++q) {
-        const DevPoints& D = rec[q];
-        std::vector<float> buf((size_t)nt * std::max(D.npts, 1));
-        HIPTRY(c, hipMemcpy(buf.data(), D.rec, sizeof(float) * buf.size(), hipMemcpyDeviceToHost));
-        float* o = shot->records[q];
-        for (int r = 0; r < D.npts; ++r)
-            for (int64_t it = 0; it < nt; ++it) o[(size_t)r * nt + it] = buf[(size_t)it * D.npts + r];
-    }
+    if ((rc = copy_records(c, rec, nt, shot->records))) return rc;
     for (int q = 0; q < shot->nfinal; ++q) {
         const int f = shot->final_fields[q];
         int64_t n3[3];
@@ -1107,14 +1123,7 @@ int gpi_born(gpi_ctx* c, uint64_t model, const gpi_shot* shot, const gpi_array* 
         smp.done(it);
     }
     if ((rc = finish_timing(c, smp, nt, launches))) return rc;
-    for (int q = 0; q < shot->nrec; ++q) {
-        const DevPoints& D = rec[q];
-        std::vector<float> buf((size_t)nt * std::max(D.npts, 1));
-        HIPTRY(c, hipMemcpy(buf.data(), D.rec, sizeof(float) * buf.size(), hipMemcpyDeviceToHost));
-        float* o = shot->records[q];
-        for (int r = 0; r < D.npts; ++r)
-            for (int64_t it = 0; it < nt; ++it) o[(size_t)r * nt + it] = buf[(size_t)it * D.npts + r];
-    }
+    if ((rc = copy_records(c, rec, nt, shot->records))) return rc;
     return GPI_OK;
 }
 

@@ -61,7 +61,25 @@ __global__ __launch_bounds__(256) void inject_kernel(InjectParams P)
     }
 }
 
+// Records of one receiver set, nt x npts point fastest on the device, into the caller's
+// layout (time fastest): lanes run along time, so the stores are coalesced (a few tens of
+// microseconds for the largest record sets, against milliseconds for a host transpose).
+__global__ __launch_bounds__(256) void records_transpose_kernel(const float* __restrict__ src, float* __restrict__ dst,
+                                                                int64_t nt, int np)
+{
+    const int64_t it = (int64_t)blockIdx.y * 256 + threadIdx.x;
+    const int r = (int)blockIdx.x;
+    if (it < nt) dst[(int64_t)r * nt + it] = src[it * np + r];
+}
+
 }  // namespace
+
+void records_transpose(const float* rec, float* out, int64_t nt, int npts, hipStream_t s)
+{
+    if (nt <= 0 || npts <= 0) return;
+    dim3 grd((unsigned)npts, (unsigned)((nt + 255) / 256)), blk(256);
+    hipLaunchKernelGGL(records_transpose_kernel, grd, blk, 0, s, rec, out, nt, npts);
+}
 
 void inject(const Model& m, const State& st, const DevPoints& pts, int64_t it, hipStream_t s)
 {
