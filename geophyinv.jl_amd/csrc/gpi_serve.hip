@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
@@ -70,6 +71,7 @@ struct Slot {
     int64_t start = 0;
     float* pin = nullptr;
     size_t pin_cap = 0;
+    float* trec = nullptr;   // device staging of the records in the caller's layout
     hipEvent_t ev = nullptr;
 };
 
@@ -116,7 +118,7 @@ uint64_t estimate_bytes(const Req& r)
     const uint64_t psi = nder * (uint64_t)(2 * g.npml) * (uint64_t)(std::max(g.n[0], g.n[2]) + ld) * sizeof(float);
     uint64_t pts = 0;
     for (const auto& p : r.src) pts += (uint64_t)p.npts * (uint64_t)(r.nt + 64) * sizeof(float);
-    for (const auto& p : r.rec) pts += (uint64_t)p.npts * (uint64_t)(r.nt + 16) * sizeof(float);
+    for (const auto& p : r.rec) pts += 2 * (uint64_t)p.npts * (uint64_t)(r.nt + 16) * sizeof(float);   // + staging
     const uint64_t buckets = 2 * ((uint64_t)(g.n[0] / 64 + 1) * (uint64_t)g.n[2] + 1) * sizeof(int32_t) * r.src.size();
     return (arrays * slab + psi + pts + buckets) * 21 / 20 + (1 << 20);
 }
@@ -369,6 +371,7 @@ int gpi_server::admit(std::unique_ptr<Req> r, Slot** out)
     if ((rc = zero_state(ac, *s->m, *s->m->st0, ac->stream))) return bail(rc);
     size_t nrec = 0;
     for (const DevPoints& D : s->rec) nrec += (size_t)q.nt * (size_t)std::max(D.npts, 1);
+    if (nrec && (rc = s->S->alloc(&s->trec, nrec))) return bail(rc);
     s->pin = pool.get(nrec, &s->pin_cap);
     if (!s->pin) return bail(fail(ac, GPI_ENOMEM, "pinned host memory for the records"));
     if (hipEventCreateWithFlags(&s->ev, hipEventDisableTiming) != hipSuccess) {
@@ -381,13 +384,15 @@ int gpi_server::admit(std::unique_ptr<Req> r, Slot** out)
 
 void gpi_server::retire(Slot* s)
 {
-    // records -> pinned staging; the device memory goes to the graveyard
+    // records -> the caller's time-fastest layout on the device -> pinned staging; the
+    // device memory goes to the graveyard
     size_t off = 0;
     for (const DevPoints& D : s->rec) {
         const size_t n = (size_t)s->r->nt * (size_t)std::max(D.npts, 1);
-        (void)hipMemcpyAsync(s->pin + off, D.rec, sizeof(float) * n, hipMemcpyDeviceToHost, cc->stream);
+        records_transpose(D.rec, s->trec + off, s->r->nt, D.npts, cc->stream);
         off += n;
     }
+    if (off) (void)hipMemcpyAsync(s->pin, s->trec, sizeof(float) * off, hipMemcpyDeviceToHost, cc->stream);
     (void)hipEventRecord(s->ev, cc->stream);
     free_allocs(cc, s->S->p, s->S->bytes);
     delete s->S;
@@ -424,20 +429,12 @@ void gpi_server::run_completions()
 void gpi_server::finalize(Slot* s, int code)
 {
     const Req& q = *s->r;
-    if (code == GPI_OK) {   // staging holds [it][r] per field, the caller [r][it]: 32 x 32 blocks
+    if (code == GPI_OK) {   // the staging already holds each field in the caller's layout
         const double t0 = now_ms();
         size_t off = 0;
         for (size_t k = 0; k < s->rec.size(); ++k) {
             const int np = s->rec[k].npts;
-            float* o = q.out[k];
-            const float* in = s->pin + off;
-            for (int64_t i0 = 0; i0 < q.nt; i0 += 32)
-                for (int r0 = 0; r0 < np; r0 += 32) {
-                    const int64_t i1 = std::min<int64_t>(i0 + 32, q.nt);
-                    const int r1 = std::min(r0 + 32, np);
-                    for (int r = r0; r < r1; ++r)
-                        for (int64_t it = i0; it < i1; ++it) o[(size_t)r * q.nt + it] = in[(size_t)it * np + r];
-                }
+            if (np > 0) std::memcpy(q.out[k], s->pin + off, sizeof(float) * (size_t)q.nt * (size_t)np);
             off += (size_t)q.nt * (size_t)std::max(np, 1);
         }
         std::lock_guard<std::mutex> lk(mu);
