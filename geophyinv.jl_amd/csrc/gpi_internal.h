@@ -139,6 +139,8 @@ struct Ctx {
 // stand-alone sparse kernels (sparse.hip): injection after a pass, for source sets that
 // cannot ride along with the pass (targets on a free-surface row or dense target lists)
 void inject(const Model& m, const State& st, const DevPoints& pts, int64_t it, hipStream_t s);
+// illumination: acc[o] += p[o]^2 over n floats (FP32 square, FP64 sum)
+void illum_accumulate(const float* p, double* acc, int64_t n, hipStream_t s);
 // receiver records, nt x npts point fastest -> npts x nt time fastest (the host layout)
 void records_transpose(const float* rec, float* out, int64_t nt, int npts, hipStream_t s);
 

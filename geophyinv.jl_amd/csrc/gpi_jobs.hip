@@ -236,9 +236,9 @@ int gpi_jobs_submit(gpi_jobs* q, const gpi_job* job, uint64_t* ticket)
         q->err = msg;
         return GPI_EINVAL;
     };
-    if (j.shot.nt < 1 || j.shot.nfinal != 0 || j.shot.nsnap != 0 || j.shot.nrec < 0 || j.shot.nsrc < 0 ||
+    if (j.shot.nt < 1 || j.shot.nfinal != 0 || j.shot.nsnap != 0 || j.shot.illum || j.shot.nrec < 0 || j.shot.nsrc < 0 ||
         (j.shot.nrec > 0 && (!j.shot.rec || !j.shot.records)) || (j.shot.nsrc > 0 && (!j.shot.src || !j.shot.wavelets)))
-        return bad("bad shot (nt, points, records; snapshots and final fields are not served)");
+        return bad("bad shot (nt, points, records; snapshots, final fields and illumination are not served)");
     if (j.checkpoint < 0) return bad("checkpoint must be >= 0");
     if (j.checkpoint > 0) {
         if (j.shot.nrec < 1 || !j.dobs || !j.grad[0] || (j.grid.physics == 1 && (!j.grad[1] || !j.grad[2])))

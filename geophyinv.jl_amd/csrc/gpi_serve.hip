@@ -835,7 +835,7 @@ int gpi_submit(gpi_server* v, const gpi_request* rq, uint64_t* ticket)
     };
     if (g.ndim != 2 || g.physics != v->o.physics || g.order != v->o.order)
         return bad("the request's dimension, physics or order differs from the server's");
-    if (sh.nt < 1 || sh.nsrc < 0 || sh.nrec < 0 || sh.nsrc > 4 || sh.nrec > 4 || sh.nfinal != 0 || sh.nsnap != 0 ||
+    if (sh.nt < 1 || sh.nsrc < 0 || sh.nrec < 0 || sh.nsrc > 4 || sh.nrec > 4 || sh.nfinal != 0 || sh.nsnap != 0 || sh.illum ||
         (sh.nsrc && (!sh.src || !sh.wavelets)) || (sh.nrec && (!sh.rec || !sh.records)))
         return bad("bad shot description");
     const gpi_medium& md = rq->medium;

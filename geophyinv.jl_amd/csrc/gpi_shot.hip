@@ -391,6 +391,8 @@ int finish_timing(gpi_ctx* c, const Sampler& smp, int64_t nt, int64_t launches)
     return GPI_OK;
 }
 
+int download_pitched_f64(gpi_ctx* c, const Model& m, double* dst, const double* src, const int64_t n[3]);
+
 // Records of every receiver set into the caller's buffers (time fastest): transposed on
 // the device into one staging buffer, then copied; the stream has drained on return.
 int copy_records(gpi_ctx* c, const std::vector<DevPoints>& rec, int64_t nt, float* const* out)
@@ -455,6 +457,14 @@ int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
         }
     }
 
+    // illumination (compute_illum!, src/fdtd/fdtd.jl:570-581): acoustic p only
+    double* illum = nullptr;
+    if (shot->illum) {
+        if (g.physics != 0) return fail(c, GPI_EINVAL, "illumination is of the acoustic pressure only");
+        if ((rc = S.alloc(&illum, (size_t)m.slab))) return rc;
+        HIPTRY(c, hipMemsetAsync(illum, 0, sizeof(double) * (size_t)m.slab, c->stream));
+    }
+
     // reset wavefields and CPML memory (reset_w2!, types.jl:101-113)
     if ((rc = zero_state(c, m, st, c->stream))) return rc;
 
@@ -482,6 +492,10 @@ int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
                 if ((rc = copy_state(c, m, st, save->ckpt[k - 1], true, c->stream))) return rc;
         } else if (save) {
             boundary_strips(m, st, *save, it, true, c->stream, &launches);
+        }
+        if (illum) {   // after the step's boundary saving and imaging, as propagate.jl:236
+            illum_accumulate(st.field[GPI_P], illum, m.slab, c->stream);
+            ++launches;
         }
         for (int q = 0; q < shot->nsnap; ++q)
             if (shot->snap_steps[q] == it)
@@ -515,6 +529,11 @@ int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
         int64_t n3[3];
         shape_of(g.ndim, g.order, g.physics, g.n, shot->snap_field, n3);
         if ((rc = download_pitched(c, m, shot->snap_out[q], snap[q], n3))) return rc;
+    }
+    if (illum) {
+        int64_t n3[3];
+        shape_of(g.ndim, g.order, g.physics, g.n, GPI_P, n3);
+        if ((rc = download_pitched_f64(c, m, shot->illum, illum, n3))) return rc;
     }
     return GPI_OK;
 }

@@ -165,6 +165,16 @@ __global__ __launch_bounds__(256) void image_exact_kernel(ImageExactParams P)
 // the reference's 2 (dobs - dcal) = -dJ/dv (database.jl:371-380), so the adjoint run holds
 // -(tau~, v~) and every sum enters negated.
 
+// compute_illum!: illum += abs2(p) with the square in FP32 and the sum in FP64 (the slab's
+// padding lanes of p stay zero, so the whole slab is summed)
+__global__ __launch_bounds__(256) void illum_kernel(const float* __restrict__ p, double* __restrict__ acc, int64_t n)
+{
+    const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (o >= n) return;
+    const float v = p[o];
+    acc[o] += (double)(v * v);
+}
+
 struct ElasticParams {
     ElasticImage E;
     int nz, ny, nx, ld, py, h;
@@ -339,6 +349,11 @@ void source_correction(const Model& m, const DevPoints& pts, const double* acc, 
     hipLaunchKernelGGL(source_correction_kernel, dim3((unsigned)((pts.ntarget + 255) / 256)), dim3(256), 0, s,
                        (const int64_t*)pts.toff, pts.ntarget, acc, (const float*)m.coef[GPI_C_INVLAMBDA],
                        (const float*)m.coef[GPI_C_INVMU], (double)m.g.dt, (double)nn, g_il, g_im);
+}
+
+void illum_accumulate(const float* p, double* acc, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(illum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, acc, n);
 }
 
 void image_acoustic_exact(const Model& m, const ImageSets& S, double* g_invK, double* g_rho, float dtI, hipStream_t s)

@@ -54,7 +54,8 @@ class gpi_shot(C.Structure):
                 ("final_out", C.POINTER(C.POINTER(C.c_float))),
                 ("snap_field", C.c_int32), ("reserved", C.c_int32),
                 ("snap_steps", C.POINTER(C.c_int64)),
-                ("snap_out", C.POINTER(C.POINTER(C.c_float)))]
+                ("snap_out", C.POINTER(C.POINTER(C.c_float))),
+                ("illum", C.POINTER(C.c_double))]
 
 
 class gpi_adjoint_shot(C.Structure):

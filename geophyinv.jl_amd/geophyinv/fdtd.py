@@ -125,7 +125,7 @@ class SeisForwExpt:
                  rfields=("vz",), pml_faces=None, rigid_faces=None, stressfree_faces=(),
                  order: int = 2, npml: int | None = None, compat: int = 1, verbose: bool = False,
                  context: "_lib.Context | None" = None, coefficients: str = "host", tsnaps=None,
-                 snaps_field=None, checkpoint: int = 0, grad_rho: bool = True):
+                 snaps_field=None, checkpoint: int = 0, grad_rho: bool = True, illum_flag: bool = False):
         ndim = medium.ndim
         if order not in (2, 4, 6, 8):
             raise ValueError("order must be 2, 4, 6 or 8")
@@ -169,9 +169,18 @@ class SeisForwExpt:
             raise ValueError("elastic gradients take velocity records only (rfields vx, vy, vz)")
         self.checkpoint = int(checkpoint)
         self.grad_rho = bool(grad_rho)
+        # source illumination (illum_flag, src/fdtd/fdtd.jl:59,73,177): sum over the time steps
+        # of p^2 of the first wavefield, stacked over the supersources on the unpadded grid
+        if illum_flag and physics != 0:
+            raise ValueError("illumination is of the acoustic pressure field")
+        self.illum_flag = bool(illum_flag)
+        self.illum_stack = np.zeros(tuple(len(g) for g in medium.grid), np.float64, order="F") if illum_flag else None
+        self.illums = [None] * len(ageom)
         self.born = bool(attrib_mod.born)
         if self.born and (physics != 0 or attrib_mod.npw != 2):
             raise ValueError("Born modelling is acoustic, with two wavefields")
+        if self.born and illum_flag:
+            raise ValueError("illumination is produced by the :forward and :forward_save runs, not Born")
         self.dmod = None          # Born: perturbation of mod (invK, rho) on the padded grid
         self.born_dcoefs = None   # Born: linearised coefficient perturbations
         self.attrib_mod = attrib_mod
@@ -485,6 +494,10 @@ class SeisForwExpt:
             shot.snap_steps = steps.ctypes.data_as(C.POINTER(C.c_int64))
             shot.snap_out = (C.POINTER(C.c_float) * len(snap_out))(*[_lib.fptr(a) for a in snap_out])
             self.snaps[iss] = snap_out
+        if self.illum_flag:
+            ill = np.zeros(field_shape(self.physics, self.ndim, self.order, self.ext_shape, "p"), np.float64, order="F")
+            shot.illum = ill.ctypes.data_as(C.POINTER(C.c_double))
+            self.illums[iss] = ill
         if save:
             self._free_store(iss)
             h = C.c_uint64()
@@ -569,10 +582,14 @@ class SeisForwExpt:
                     if k in self.gradients:
                         self.gradients[k][...] = self.gradients[k] + g
             return self.timing
+        if self.illum_flag:   # initialize! zeroes the stack (src/fdtd/types.jl:171)
+            self.illum_stack.fill(0.0)
         for iss in range(len(self.ageom)):
             recs, _ = self.run_shot(iss, save=(mode == "forward_save"))
             for f, r in recs.items():
                 self.data[iss].d[f][:, :] = r
+            if self.illum_flag:   # stack_illums! (src/fdtd/fdtd.jl:556-566)
+                self.illum_stack += self.illums[iss][self.view_inner(self.illums[iss])]
         return self.timing
 
     # ------------------------------------------------------------------ model vectors
@@ -740,6 +757,8 @@ class SeisForwExpt:
             return self.data if iss is None else self.data[iss]
         if key == "snaps":   # pa[:snaps, i] (src/fdtd/getprop.jl:13-24)
             return self.snaps[0 if iss is None else iss]
+        if key == "illum_stack":   # pac.illum_stack (src/fdtd/types.jl:155)
+            return self.illum_stack
         if key in ("ageom", "srcwav", "medium", "exmedium", "rfields", "pml_faces", "rigid_faces", "gradients",
                    "mod", "ref_mod"):
             return getattr(self, key)

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define GPI_ABI_VERSION 4
+#define GPI_ABI_VERSION 5
 
 /* error codes */
 #define GPI_OK 0
@@ -133,6 +133,9 @@ typedef struct gpi_shot {
     int32_t reserved;
     const int64_t* snap_steps;       /* nsnap zero-based steps: the field after that step */
     float* const* snap_out;          /* nsnap host outputs of the field's full size */
+    double* illum;                   /* optional (acoustic): sum over the steps of p^2 after each
+                                        step, FP32 square and FP64 sum (compute_illum!,
+                                        src/fdtd/fdtd.jl:570-581), host output on the stress grid */
 } gpi_shot;
 
 /* Timing of the last gpi_forward call on a context (device time, HIP events on the

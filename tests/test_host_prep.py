@@ -102,3 +102,22 @@ def test_vectorised_weights_equal_pointwise():
             for k, p in enumerate(pts):
                 i, w = point_weights(g, p, compat)
                 assert np.array_equal(I[k], i) and np.array_equal(W[k], w), (nd, compat, k)
+
+
+def test_illum_flag_checks():
+    """illum_flag: the stack has the unpadded grid's shape and starts at zero; elastic media
+    (no pressure field) and Born modelling reject it."""
+    med = layered_acoustic(nz=30, nx=34)
+    base = experiment(med, nt=120)
+    pa = G.SeisForwExpt(base.attrib_mod, medium=med, tgrid=base.tgrid, ageom=base.ageom, srcwav=base.srcwav,
+                        rfields=base.rfields, npml=base.npml, illum_flag=True)
+    assert pa["illum_stack"].shape == (30, 34) and pa["illum_stack"].dtype == np.float64
+    assert not pa["illum_stack"].any()
+    emed = layered_elastic(nz=30, nx=34)
+    eb = experiment(emed, nt=120, sfields=("tauxx",), rfields=("vz",))
+    with pytest.raises(ValueError):
+        G.SeisForwExpt(eb.attrib_mod, medium=emed, tgrid=eb.tgrid, ageom=eb.ageom, srcwav=eb.srcwav,
+                       rfields=eb.rfields, npml=eb.npml, illum_flag=True)
+    with pytest.raises(ValueError):
+        G.SeisForwExpt(G.FdtdAcoustic[G.Born](), medium=med, tgrid=base.tgrid, ageom=base.ageom,
+                       srcwav=base.srcwav, rfields=base.rfields, npml=base.npml, illum_flag=True)
