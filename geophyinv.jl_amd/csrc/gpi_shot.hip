@@ -391,8 +391,6 @@ int finish_timing(gpi_ctx* c, const Sampler& smp, int64_t nt, int64_t launches)
     return GPI_OK;
 }
 
-int download_pitched_f64(gpi_ctx* c, const Model& m, double* dst, const double* src, const int64_t n[3]);
-
 // Records of every receiver set into the caller's buffers (time fastest): transposed on
 // the device into one staging buffer, then copied; the stream has drained on return.
 int copy_records(gpi_ctx* c, const std::vector<DevPoints>& rec, int64_t nt, float* const* out)
@@ -533,7 +531,7 @@ int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
     if (illum) {
         int64_t n3[3];
         shape_of(g.ndim, g.order, g.physics, g.n, GPI_P, n3);
-        if ((rc = download_pitched_f64(c, m, shot->illum, illum, n3))) return rc;
+        if ((rc = download_pitched(c, m, shot->illum, illum, n3))) return rc;
     }
     return GPI_OK;
 }
@@ -553,18 +551,6 @@ int check_shot(gpi_ctx* c, uint64_t model, const gpi_shot* shot, Model** out)
     return GPI_OK;
 }
 
-// FP64 pitched slab -> compact host array of logical size n[3]
-int download_pitched_f64(gpi_ctx* c, const Model& m, double* dst, const double* src, const int64_t n[3])
-{
-    std::vector<double> stage((size_t)m.slab);
-    HIPTRY(c, hipStreamSynchronize(c->stream));
-    HIPTRY(c, hipMemcpy(stage.data(), src, sizeof(double) * (size_t)m.slab, hipMemcpyDeviceToHost));
-    for (int64_t ix = 0; ix < n[2]; ++ix)
-        for (int64_t iy = 0; iy < n[1]; ++iy)
-            std::memcpy(dst + n[0] * (iy + n[1] * ix), &stage[(size_t)(m.pitch0 * (iy + m.pitch1 * ix))],
-                        sizeof(double) * (size_t)n[0]);
-    return GPI_OK;
-}
 
 }  // namespace
 
@@ -867,8 +853,8 @@ int gpi_adjoint_checkpoint(gpi_ctx* c, uint64_t model, uint64_t store, const gpi
     if ((rc = finish_timing(c, smp, nt, launches))) return rc;
     int64_t n3[3];
     shape_of(g.ndim, g.order, g.physics, g.n, GPI_P, n3);
-    if ((rc = download_pitched_f64(c, m, g_invK, gK, n3))) return rc;
-    if (rho && (rc = download_pitched_f64(c, m, g_rho, gR, n3))) return rc;
+    if ((rc = download_pitched(c, m, g_invK, gK, n3))) return rc;
+    if (rho && (rc = download_pitched(c, m, g_rho, gR, n3))) return rc;
     return GPI_OK;
 }
 
@@ -1062,7 +1048,7 @@ int gpi_adjoint_checkpoint_elastic(gpi_ctx* c, uint64_t model, uint64_t store, c
     shape_of(g.ndim, g.order, g.physics, g.n, GPI_TAUXX, n3);
     double* host[3] = {g_invlambda, g_invmu, g_rho};
     for (int q = 0; q < 3; ++q)
-        if ((rc = download_pitched_f64(c, m, host[q], gout[q], n3))) return rc;
+        if ((rc = download_pitched(c, m, host[q], gout[q], n3))) return rc;
     return GPI_OK;
 }
 

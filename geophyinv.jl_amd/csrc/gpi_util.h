@@ -108,7 +108,9 @@ int prep_coefficients_2d(gpi_ctx* c, Model& m, const float* vp, const float* vs,
 int copy_h2d(gpi_ctx* c, void* dst, const void* src, size_t bytes);
 
 int upload_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3]);
+// pitched device slab -> compact host array of logical size n[3] (blocking)
 int download_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3]);
+int download_pitched(gpi_ctx* c, const Model& m, double* dst, const double* src, const int64_t n[3]);
 
 // Scratch device buffers of one call, freed on every exit path (after the stream drained).
 struct Scratch {

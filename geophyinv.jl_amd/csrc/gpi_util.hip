@@ -367,16 +367,44 @@ int upload_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, con
     return copy_h2d(c, dst, stage.data(), sizeof(float) * (size_t)m.slab);
 }
 
+// pitched slab -> compact array of n0 x n1 x n2 (grid: x over n0 in 256s, y over n1, z over n2)
+template <typename T>
+__global__ __launch_bounds__(256) void pack_pitched_kernel(const T* __restrict__ src, T* __restrict__ dst,
+                                                           int n0, int n1, int64_t pitch0, int64_t pitch1)
+{
+    const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (i >= n0) return;
+    const int64_t y = blockIdx.y, z = blockIdx.z;
+    dst[i + (int64_t)n0 * (y + (int64_t)n1 * z)] = src[i + pitch0 * (y + pitch1 * z)];
+}
+
+// Packed on the device, then one copy into the caller's array (no host-side staging slab).
+template <typename T>
+static int download_packed(gpi_ctx* c, const Model& m, T* dst, const T* src, const int64_t n[3])
+{
+    if (n[0] < 1 || n[1] < 1 || n[2] < 1 || n[0] > m.pitch0 || n[1] > 65535 || n[2] > 65535)
+        return fail(c, GPI_EINVAL, "download: bad field size");
+    const size_t cnt = (size_t)(n[0] * n[1] * n[2]);
+    Scratch S(c);
+    T* tmp = nullptr;
+    int rc = S.alloc(&tmp, cnt);
+    if (rc) return rc;
+    hipLaunchKernelGGL(pack_pitched_kernel<T>, dim3((unsigned)((n[0] + 255) / 256), (unsigned)n[1], (unsigned)n[2]),
+                       dim3(256), 0, c->stream, src, tmp, (int)n[0], (int)n[1], m.pitch0, m.pitch1);
+    HIPTRY(c, hipGetLastError());
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    HIPTRY(c, hipMemcpy(dst, tmp, sizeof(T) * cnt, hipMemcpyDeviceToHost));
+    return GPI_OK;
+}
+
 int download_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3])
 {
-    std::vector<float> stage((size_t)m.slab);
-    HIPTRY(c, hipStreamSynchronize(c->stream));
-    HIPTRY(c, hipMemcpy(stage.data(), src, sizeof(float) * (size_t)m.slab, hipMemcpyDeviceToHost));
-    for (int64_t ix = 0; ix < n[2]; ++ix)
-        for (int64_t iy = 0; iy < n[1]; ++iy)
-            std::memcpy(dst + n[0] * (iy + n[1] * ix), &stage[(size_t)(m.pitch0 * (iy + m.pitch1 * ix))],
-                        sizeof(float) * (size_t)n[0]);
-    return GPI_OK;
+    return download_packed(c, m, dst, src, n);
+}
+
+int download_pitched(gpi_ctx* c, const Model& m, double* dst, const double* src, const int64_t n[3])
+{
+    return download_packed(c, m, dst, src, n);
 }
 
 }  // namespace gpi

@@ -35,6 +35,13 @@ class _Data:
     def copy(self):
         return type(self)(self.n, self.grid, self.fields, {f: a.copy() for f, a in self.d.items()})
 
+    def reversed(self):
+        """The same data with time reversed, as views (no copy); read-only use."""
+        r = object.__new__(type(self))
+        r.__dict__.update(self.__dict__)
+        r.d = {f: a[::-1] for f, a in self.d.items()}
+        return r
+
     def fill(self, value: float = 0.0):
         for a in self.d.values():
             a.fill(value)

@@ -316,7 +316,10 @@ class SeisForwExpt:
                     continue
                 full = np.zeros((nt, s.n), F32, order="F")
                 snt = min(len(s.grid), nt)
-                full[:snt] = get_source(s.d[f], f, self.src_types[1])[:snt]
+                if self.src_types[1] == 1:     # get_source's copy, straight into place
+                    full[:snt] = s.d[f][:snt]
+                elif self.src_types[1] != 0:   # type 0: zeros
+                    full[:snt] = get_source(s.d[f], f, self.src_types[1])[:snt]
                 per[f] = full
             self.wavelets2.append(per)
 
@@ -443,11 +446,12 @@ class SeisForwExpt:
         return ((_lib.gpi_points * max(1, len(pts)))(*pts), (C.POINTER(C.c_float) * max(1, len(wavs)))(*wavs),
                 len(pts))
 
-    def run_shot(self, iss: int, final_fields=(), save: bool = False, snaps: bool = True):
+    def run_shot(self, iss: int, final_fields=(), save: bool = False, snaps: bool = True, into=None):
         """Propagate supersource ``iss`` on the GPU; returns (records dict, finals dict).
 
         ``save`` runs the :forward_save variant, which keeps the boundary strips and the
-        final snapshots of this supersource resident for a later :adjoint run."""
+        final snapshots of this supersource resident for a later :adjoint run.  ``into``
+        (field -> FP32 Fortran-ordered nt x nr array) receives the records in place."""
         self._ensure_model()
         ctx = self._ctx
         nt = len(self.tgrid)
@@ -457,7 +461,10 @@ class SeisForwExpt:
         for f, (idx, w) in self.interp[iss].items():
             idx = np.ascontiguousarray(idx)
             w = np.ascontiguousarray(w)
-            out = np.zeros((nt, idx.shape[0]), F32, order="F")
+            out = None if into is None else into.get(f)
+            if (out is None or out.shape != (nt, idx.shape[0]) or out.dtype != F32
+                    or not out.flags.f_contiguous or not out.flags.writeable):
+                out = np.zeros((nt, idx.shape[0]), F32, order="F")
             keep += [idx, w, out]
             p = _lib.gpi_points()
             p.field, p.npts, p.nnz = FIELD_ID[f], idx.shape[0], idx.shape[1]
@@ -580,14 +587,15 @@ class SeisForwExpt:
                 gi = self.run_adjoint(iss)
                 for k, g in gi.items():
                     if k in self.gradients:
-                        self.gradients[k][...] = self.gradients[k] + g
+                        np.add(self.gradients[k], g, out=self.gradients[k])
             return self.timing
         if self.illum_flag:   # initialize! zeroes the stack (src/fdtd/types.jl:171)
             self.illum_stack.fill(0.0)
         for iss in range(len(self.ageom)):
-            recs, _ = self.run_shot(iss, save=(mode == "forward_save"))
+            recs, _ = self.run_shot(iss, save=(mode == "forward_save"), into=self.data[iss].d)
             for f, r in recs.items():
-                self.data[iss].d[f][:, :] = r
+                if r is not self.data[iss].d[f]:
+                    self.data[iss].d[f][:, :] = r
             if self.illum_flag:   # stack_illums! (src/fdtd/fdtd.jl:556-566)
                 self.illum_stack += self.illums[iss][self.view_inner(self.illums[iss])]
         return self.timing

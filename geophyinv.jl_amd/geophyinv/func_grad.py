@@ -58,11 +58,9 @@ def backprop(g, m, loss, dobs, pa, mparams=None) -> None:
     sources and write the chain-ruled gradient into ``g``.  Leaves ``pa`` in the :adjoint
     mode with its data zeroed, as the reference's ``update!`` does."""
     mparams = list(mparams or pa.mparams)
-    # adjoint sources, time reversed
-    database.gradient(pa.srcwav2, loss, dobs, pa.data)
-    for S in pa.srcwav2:
-        for f in S.fields:
-            S.d[f][:, :] = S.d[f][::-1].copy()
+    # adjoint sources, time reversed: gradient!(buffer, loss, dobs, dcal) of the time-reversed
+    # records, the same FP32 values as reversing its output, in one pass
+    database.gradient(pa.srcwav2, loss, [d.reversed() for d in dobs], [d.reversed() for d in pa.data])
     # the reference's first wavefield runs backwards with the type -1 sink (velocity sources
     # only); the checkpointed adjoint recomputes it from the stored run's own sources, so
     # any source field works there and the first wavelets are left as they are
@@ -70,14 +68,19 @@ def backprop(g, m, loss, dobs, pa, mparams=None) -> None:
     pa.attrib_mod.mode = "adjoint"
     pa.update()
     x = np.split(np.asarray(m, np.float64), len(mparams))
-    out = []
-    for xi, name in zip(x, mparams):
+    flat = g.reshape(-1)
+    if not np.shares_memory(flat, g):
+        flat = np.empty(g.size, g.dtype)
+    gs = np.split(flat, len(mparams))
+    for xi, gi, name in zip(x, gs, mparams):
         gm = pa.gradients[name][pa.view_inner(pa.gradients[name])]
-        xr = xi.reshape(gm.shape, order="F")
         # gm1 * exp(x1) * r in FP64, stored back into the FP32 gradient
-        gm32 = (gm.astype(np.float64) * np.exp(xr) * np.float64(pa.ref_mod[name])).astype(F32)
-        out.append(gm32.ravel(order="F"))
-    g[...] = np.concatenate(out)
+        t = np.exp(xi.reshape(gm.shape, order="F"))
+        t *= gm
+        t *= np.float64(pa.ref_mod[name])
+        gi.reshape(gm.shape, order="F")[...] = t if gi.dtype == F32 else t.astype(F32)
+    if not np.shares_memory(flat, g):
+        g[...] = flat.reshape(g.shape)
 
 
 def chainrule_elastic(g_invlambda, g_invmu, g_rho, vp, vs, rho):
