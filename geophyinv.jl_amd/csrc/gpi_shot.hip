@@ -551,7 +551,6 @@ int check_shot(gpi_ctx* c, uint64_t model, const gpi_shot* shot, Model** out)
     return GPI_OK;
 }
 
-
 }  // namespace
 
 extern "C" {

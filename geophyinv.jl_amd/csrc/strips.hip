@@ -165,8 +165,8 @@ __global__ __launch_bounds__(256) void image_exact_kernel(ImageExactParams P)
 // the reference's 2 (dobs - dcal) = -dJ/dv (database.jl:371-380), so the adjoint run holds
 // -(tau~, v~) and every sum enters negated.
 
-// compute_illum!: illum += abs2(p) with the square in FP32 and the sum in FP64 (the slab's
-// padding lanes of p stay zero, so the whole slab is summed)
+// compute_illum!: illum += abs2(p) with the square in FP32 and the sum in FP64, over the
+// whole pitched slab (the padding lanes are summed too and never read back)
 __global__ __launch_bounds__(256) void illum_kernel(const float* __restrict__ p, double* __restrict__ acc, int64_t n)
 {
     const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
