@@ -166,7 +166,7 @@ class SeisInvExpt:
 
         def fg(m):
             J = self.gradient(g, m)
-            return J, g.astype(np.float64).copy()
+            return J, g.astype(np.float64)   # a new FP64 array per call
 
         res = minimize(fg, m0, jac=True, method="L-BFGS-B", callback=callback,
                        options={"maxiter": maxiter, "gtol": 1e-25, "ftol": 0.0})
