@@ -1244,10 +1244,18 @@ int oracle_get_memory(const oracle_t* S, int f, float* out, int64_t n)
     return 0;
 }
 
+/* coefficient array c; O_C_RHO is the padded density (mod[:rho]) the velocity sources read */
+static const arr_t* coef_array(const oracle_t* S, int c)
+{
+    if (c == O_C_RHO && S->rho.a) return &S->rho;
+    return (c >= 0 && c < O_NCOEF && S->coef[c].a) ? &S->coef[c] : NULL;
+}
+
 int oracle_get_coef(const oracle_t* S, int c, float* out, int64_t n)
 {
-    if (c < 0 || c >= O_NCOEF || !S->coef[c].a || numel(&S->coef[c]) != n) return -1;
-    memcpy(out, S->coef[c].a, sizeof(float) * (size_t)n);
+    const arr_t* A = coef_array(S, c);
+    if (!A || numel(A) != n) return -1;
+    memcpy(out, A->a, sizeof(float) * (size_t)n);
     return 0;
 }
 
@@ -1263,8 +1271,9 @@ int oracle_set_coef(oracle_t* S, int c, const float* in, int64_t n)
 
 int oracle_coef_shape(const oracle_t* S, int c, int64_t out[3])
 {
-    if (c < 0 || c >= O_NCOEF || !S->coef[c].a) return -1;
-    for (int a = 0; a < 3; ++a) out[a] = S->coef[c].n[a];
+    const arr_t* A = coef_array(S, c);
+    if (!A) return -1;
+    for (int a = 0; a < 3; ++a) out[a] = A->n[a];
     return 0;
 }
 
