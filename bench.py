@@ -265,7 +265,7 @@ def run_gradient3d(args, G, _lib, gallery):
                                "true vp 1500->4000 m/s gradient + three +-8% spheres, start = true smoothed by a "
                                "5-cell Gaussian, 12 Hz Ricker on p, 201x201 surface receivers (vz), "
                                f"checkpoints every {args.checkpoint} steps",
-                   "time_steps": nt, "grid": list(shape), "order": pa.order},
+                   "time_steps": nt, "grid": list(shape), "order": pa.order, "compat": pa.compat},
         "propagations_per_gradient": 3,
         "gradient_s": round(mean_wall, 2),
         "forward_device_ms": round(fwd_ms, 1),
@@ -332,7 +332,7 @@ def run_gradient3d_elastic(args, G, _lib, gallery):
                                f"20-cell CPML ({shape[0]}^3), three layers vp 1800/2600/3500 m/s with a seeded "
                                "dipping interface, start = interface one cell shallower, 25 Hz explosive source, "
                                f"64x64 surface receivers (vx, vy, vz), checkpoints every {args.checkpoint} steps",
-                   "time_steps": nt, "grid": list(shape), "order": pa.order},
+                   "time_steps": nt, "grid": list(shape), "order": pa.order, "compat": pa.compat},
         "propagations_per_gradient": 3,
         "gradient_s": round(mean_wall, 2),
         "forward_device_ms": round(fwd_ms, 1),
@@ -431,7 +431,7 @@ def run_serve3d(args, G, _lib, gallery):
         "config": {"workload": f"{ns} 3-D elastic shots of gallery.elastic_fwi_3d, side U{{{side[0]}..{side[1]}}} "
                                f"+ 40 CPML cells, nt U{{{steps[0]}..{steps[1]}}}, {args.grad_fraction:.0%} "
                                f"gradient requests (checkpoints every {args.checkpoint}), all queued at t = 0",
-                   "workers": args.workers},
+                   "workers": args.workers, "compat": 0},
         "gradient_shots": int(sum(sh["grad"] for sh in shots)),
         "plain": {"shots_per_s": round(ns / span_p, 3), "span_s": round(span_p, 2), "p50_ms": pct(lat_p, 50),
                   "p99_ms": pct(lat_p, 99)},
@@ -649,7 +649,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": workload, "time_steps": nt, "grid": list(shape), "order": pa.order},
+        "config": {"workload": workload, "time_steps": nt, "grid": list(shape), "order": pa.order,
+                   "compat": pa.compat},
         "device_ms_per_step": round(float(np.mean(dev_ms)), 3),
         "device_gpt_per_s": round(cells * nt / (float(np.mean(dev_ms)) * 1e-3) / 1e9, 3),
         "step_algorithmic_GBps": round(step_bytes * nt / (float(np.mean(dev_ms)) * 1e-3) / 1e9, 1),

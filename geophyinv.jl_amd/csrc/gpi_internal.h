@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <unordered_map>
@@ -90,14 +91,14 @@ struct Store {
 // Sub-allocator of device memory for contexts that allocate and free at a high rate (the
 // serving scheduler): segments from hipMalloc, carved first-fit into 256-byte aligned
 // blocks; a freed block coalesces with its free neighbours.  Not thread-safe: one thread
-// (the scheduler) allocates and frees.
+// (the scheduler) allocates and frees; reserved() may be read from any thread.
 class DevicePool {
 public:
     ~DevicePool();
     hipError_t alloc(void** p, size_t bytes);
     void free(void* p);
     void trim();                           // hipFree every segment with no live block
-    uint64_t reserved() const { return reserved_; }
+    uint64_t reserved() const { return reserved_.load(std::memory_order_relaxed); }
 
 private:
     struct Seg {
@@ -107,7 +108,7 @@ private:
     };
     std::vector<Seg*> segs_;
     std::unordered_map<void*, std::pair<Seg*, size_t>> blocks_;   // block -> (segment, size)
-    uint64_t reserved_ = 0;
+    std::atomic<uint64_t> reserved_{0};
     bool carve(Seg* g, size_t n, void** p);
 };
 
@@ -142,7 +143,8 @@ void inject(const Model& m, const State& st, const DevPoints& pts, int64_t it, h
 // illumination: acc[o] += p[o]^2 over n floats (FP32 square, FP64 sum)
 void illum_accumulate(const float* p, double* acc, int64_t n, hipStream_t s);
 // receiver records, nt x npts point fastest -> npts x nt time fastest (the host layout)
-void records_transpose(const float* rec, float* out, int64_t nt, int npts, hipStream_t s);
+// nonfinite: device int set to 1 when a record sample is Inf or NaN (not cleared here)
+void records_transpose(const float* rec, float* out, int64_t nt, int npts, int* nonfinite, hipStream_t s);
 
 // time-reversal kernels (strips.hip)
 void strip_copy(const Model& m, float* field, const int64_t fn[3], int axis, int64_t np, float* store,

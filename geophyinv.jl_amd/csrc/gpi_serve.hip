@@ -371,8 +371,9 @@ int gpi_server::admit(std::unique_ptr<Req> r, Slot** out)
     if ((rc = zero_state(ac, *s->m, *s->m->st0, ac->stream))) return bail(rc);
     size_t nrec = 0;
     for (const DevPoints& D : s->rec) nrec += (size_t)q.nt * (size_t)std::max(D.npts, 1);
-    if (nrec && (rc = s->S->alloc(&s->trec, nrec))) return bail(rc);
-    s->pin = pool.get(nrec, &s->pin_cap);
+    // staging of the transposed records plus one int: the non-finite flag of the transpose
+    if ((rc = s->S->alloc(&s->trec, nrec + 1))) return bail(rc);
+    s->pin = pool.get(nrec + 1, &s->pin_cap);
     if (!s->pin) return bail(fail(ac, GPI_ENOMEM, "pinned host memory for the records"));
     if (hipEventCreateWithFlags(&s->ev, hipEventDisableTiming) != hipSuccess) {
         pool.put(s->pin, s->pin_cap);
@@ -384,15 +385,19 @@ int gpi_server::admit(std::unique_ptr<Req> r, Slot** out)
 
 void gpi_server::retire(Slot* s)
 {
-    // records -> the caller's time-fastest layout on the device -> pinned staging; the
-    // device memory goes to the graveyard
+    // records -> the caller's time-fastest layout on the device -> pinned staging, with the
+    // non-finite flag after them; the device memory goes to the graveyard
+    size_t total = 0;
+    for (const DevPoints& D : s->rec) total += (size_t)s->r->nt * (size_t)std::max(D.npts, 1);
+    int* flag = reinterpret_cast<int*>(s->trec + total);
+    (void)hipMemsetAsync(flag, 0, sizeof(int), cc->stream);
     size_t off = 0;
     for (const DevPoints& D : s->rec) {
         const size_t n = (size_t)s->r->nt * (size_t)std::max(D.npts, 1);
-        records_transpose(D.rec, s->trec + off, s->r->nt, D.npts, cc->stream);
+        records_transpose(D.rec, s->trec + off, s->r->nt, D.npts, flag, cc->stream);
         off += n;
     }
-    if (off) (void)hipMemcpyAsync(s->pin, s->trec, sizeof(float) * off, hipMemcpyDeviceToHost, cc->stream);
+    (void)hipMemcpyAsync(s->pin, s->trec, sizeof(float) * (total + 1), hipMemcpyDeviceToHost, cc->stream);
     (void)hipEventRecord(s->ev, cc->stream);
     free_allocs(cc, s->S->p, s->S->bytes);
     delete s->S;
@@ -437,8 +442,15 @@ void gpi_server::finalize(Slot* s, int code)
             if (np > 0) std::memcpy(q.out[k], s->pin + off, sizeof(float) * (size_t)q.nt * (size_t)np);
             off += (size_t)q.nt * (size_t)std::max(np, 1);
         }
+        int nonfinite = 0;
+        std::memcpy(&nonfinite, s->pin + off, sizeof(int));
         std::lock_guard<std::mutex> lk(mu);
         t_finalize += now_ms() - t0;
+        if (nonfinite) {   // records are delivered, the shot reports the error
+            code = GPI_ENONFINITE;
+            err = "shot " + std::to_string(q.ticket) +
+                  " recorded non-finite values (is dt within the stability limit?)";
+        }
     }
     pool.put(s->pin, s->pin_cap);
     (void)hipEventDestroy(s->ev);

@@ -393,6 +393,8 @@ int finish_timing(gpi_ctx* c, const Sampler& smp, int64_t nt, int64_t launches)
 
 // Records of every receiver set into the caller's buffers (time fastest): transposed on
 // the device into one staging buffer, then copied; the stream has drained on return.
+// Returns GPI_ENONFINITE (after the copies) when a record holds Inf or NaN, as a shot
+// that violates the stability condition does (src/fdtd/stability.jl:8-60 only warns).
 int copy_records(gpi_ctx* c, const std::vector<DevPoints>& rec, int64_t nt, float* const* out)
 {
     size_t most = 0;
@@ -400,17 +402,27 @@ int copy_records(gpi_ctx* c, const std::vector<DevPoints>& rec, int64_t nt, floa
     if (!most) return GPI_OK;
     Scratch S(c);
     float* tmp = nullptr;
+    int* flag = nullptr;
     int rc = S.alloc(&tmp, most);
+    if (!rc) rc = S.alloc(&flag, 1);
     if (rc) return rc;
+    int bad = -1;
     for (size_t q = 0; q < rec.size(); ++q) {
         const DevPoints& D = rec[q];
         if (D.npts <= 0) continue;
-        records_transpose(D.rec, tmp, nt, D.npts, c->stream);
+        int nonfinite = 0;
+        HIPTRY(c, hipMemsetAsync(flag, 0, sizeof(int), c->stream));
+        records_transpose(D.rec, tmp, nt, D.npts, flag, c->stream);
         HIPTRY(c, hipGetLastError());
         HIPTRY(c, hipMemcpyAsync(out[q], tmp, sizeof(float) * (size_t)nt * (size_t)D.npts, hipMemcpyDeviceToHost,
                                  c->stream));
+        HIPTRY(c, hipMemcpyAsync(&nonfinite, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPTRY(c, hipStreamSynchronize(c->stream));   // tmp is reused by the next set
+        if (nonfinite && bad < 0) bad = (int)q;
     }
+    if (bad >= 0)
+        return fail(c, GPI_ENONFINITE, "receiver set " + std::to_string(bad) +
+                                           " recorded non-finite values (is dt within the stability limit?)");
     return GPI_OK;
 }
 
@@ -515,8 +527,11 @@ int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
                                          hipMemcpyDeviceToDevice, c->stream));
     if ((rc = finish_timing(c, smp, nt, launches))) return rc;
 
-    // copy back
-    if ((rc = copy_records(c, rec, nt, shot->records))) return rc;
+    // copy back (everything, also when the records hold non-finite values: that code is
+    // returned last, with the message copy_records left)
+    const int rec_rc = copy_records(c, rec, nt, shot->records);
+    if (rec_rc && rec_rc != GPI_ENONFINITE) return rec_rc;
+    const std::string rec_msg = c->err;
     for (int q = 0; q < shot->nfinal; ++q) {
         const int f = shot->final_fields[q];
         int64_t n3[3];
@@ -533,6 +548,7 @@ int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
         shape_of(g.ndim, g.order, g.physics, g.n, GPI_P, n3);
         if ((rc = download_pitched(c, m, shot->illum, illum, n3))) return rc;
     }
+    if (rec_rc) return fail(c, rec_rc, rec_msg);
     return GPI_OK;
 }
 

@@ -64,21 +64,27 @@ __global__ __launch_bounds__(256) void inject_kernel(InjectParams P)
 // Records of one receiver set, nt x npts point fastest on the device, into the caller's
 // layout (time fastest): lanes run along time, so the stores are coalesced (a few tens of
 // microseconds for the largest record sets, against milliseconds for a host transpose).
+// A sample whose exponent bits are all set (Inf or NaN) stores 1 into *nonfinite; every
+// such store writes the same value, so no atomic is needed.
 __global__ __launch_bounds__(256) void records_transpose_kernel(const float* __restrict__ src, float* __restrict__ dst,
-                                                                int64_t nt, int np)
+                                                                int64_t nt, int np, int* __restrict__ nonfinite)
 {
     const int64_t it = (int64_t)blockIdx.y * 256 + threadIdx.x;
     const int r = (int)blockIdx.x;
-    if (it < nt) dst[(int64_t)r * nt + it] = src[it * np + r];
+    if (it < nt) {
+        const float v = src[it * np + r];
+        dst[(int64_t)r * nt + it] = v;
+        if ((__float_as_uint(v) & 0x7f800000u) == 0x7f800000u) *nonfinite = 1;
+    }
 }
 
 }  // namespace
 
-void records_transpose(const float* rec, float* out, int64_t nt, int npts, hipStream_t s)
+void records_transpose(const float* rec, float* out, int64_t nt, int npts, int* nonfinite, hipStream_t s)
 {
     if (nt <= 0 || npts <= 0) return;
     dim3 grd((unsigned)npts, (unsigned)((nt + 255) / 256)), blk(256);
-    hipLaunchKernelGGL(records_transpose_kernel, grd, blk, 0, s, rec, out, nt, npts);
+    hipLaunchKernelGGL(records_transpose_kernel, grd, blk, 0, s, rec, out, nt, npts, nonfinite);
 }
 
 void inject(const Model& m, const State& st, const DevPoints& pts, int64_t it, hipStream_t s)

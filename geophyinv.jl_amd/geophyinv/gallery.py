@@ -135,7 +135,9 @@ def acoustic_gradient_3d(n: int = 400, ds: float = 10.0, nt: int = 3000, npml: i
     """3-D acoustic: vp = 1500 + 2500 z/L m/s with three spherical anomalies of +-8 % (centres
     in the middle half of the cube, radii 150-400 m), rho = 2000; 12 Hz Ricker on p five cells
     below the surface at the centre; nrec_side^2 surface receivers at 20 m spacing recording
-    p; dt at CFL 0.5."""
+    p; dt at CFL 0.5.  The keywords carry compat=0: each coordinate is searched on its own
+    axis, so the source and the receiver patch are where this describes them (the
+    reference's 3-D spray swaps z and x, src/proj_mat.jl:240)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     L = (n - 1) * ds
     grid = [Grid.range(0.0, L, length=n) for _ in range(3)]
@@ -161,7 +163,7 @@ def acoustic_gradient_3d(n: int = 400, ds: float = 10.0, nt: int = 3000, npml: i
                      {"z": np.zeros(ry.size), "y": ry.ravel(), "x": rx.ravel()})]
     src = Srcs(1, tgrid, ["p"])
     src.update(["p"], _long_ricker(freq, tgrid))
-    return medium, ageom, [src], tgrid, dict(rfields=["p"], npml=npml)
+    return medium, ageom, [src], tgrid, dict(rfields=["p"], npml=npml, compat=0)
 
 
 def acoustic_fwi_3d(n: int = 400, ds: float = 10.0, nt: int = 3000, npml: int = 20, freq: float = 12.0,
@@ -190,6 +192,8 @@ def elastic_fwi_3d(n: int = 256, ds: float = 5.0, nt: int = 2000, npml: int = 20
     cells below the surface at the centre; an nrec_side x nrec_side receiver patch on the
     surface spanning the model, recording vx, vy and vz; dt at CFL 0.5 of sqrt(vp^2+vs^2).
     The starting model is the same medium with the dipping interface one cell shallower.
+    The keywords carry compat=0 (coordinates on their own axes, no z<->x swap of
+    src/proj_mat.jl:240), so the source and the patch sit at the surface.
 
     Returns (true medium, starting medium, ageom, srcwav, tgrid, kw)."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -220,7 +224,7 @@ def elastic_fwi_3d(n: int = 256, ds: float = 5.0, nt: int = 2000, npml: int = 20
                      {"z": np.zeros(rx.size), "y": ry.ravel(), "x": rx.ravel()})]
     src = Srcs(1, tgrid, ["tauxx"])
     src.update(["tauxx"], _long_ricker(freq, tgrid) * 1e6)
-    kw = dict(rfields=["vx", "vy", "vz"], npml=npml)
+    kw = dict(rfields=["vx", "vy", "vz"], npml=npml, compat=0)
     return true, start, ageom, [src], tgrid, kw
 
 

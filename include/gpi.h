@@ -166,8 +166,12 @@ int gpi_field_shape(int32_t ndim, int32_t order, int32_t physics, const int64_t 
 /* Replaces the device-side state built by PFdtd (src/fdtd/fdtd.jl:135-160) and refreshed
  * by update!(pa, medium) (src/fdtd/medium.jl:128-140): coefficient arrays and CPML
  * vectors.  Required coefficients: acoustic DTINVAV{X,(Y),Z}IRHO, DTK, RHO; elastic
- * DTINVAV{X,(Y),Z}IRHO, DTM, DTLAMBDA, DTAVMU (2-D) or DTAV{XZ,XY,YZ}IMU (3-D), RHO.
- * One gpi_pml per derivative field of the physics when pml_mask != 0. */
+ * DTINVAV{X,(Y),Z}IRHO, DTM, DTLAMBDA, DTAVMU (2-D) or DTAV{XZ,XY,YZ}IMU (3-D), RHO,
+ * INVLAMBDA and INVMU (the elastic passes form their dt-scaled coefficients from rho,
+ * invlambda and invmu).  That is every pac.dmod and pac.mod array of MediumParameters
+ * (src/fdtd/medium.jl:81-95); INVK is accepted for acoustic models and kept for
+ * gpi_model_get_coef.  One gpi_pml per derivative field of the physics when
+ * pml_mask != 0. */
 int gpi_model_create(gpi_ctx* ctx, const gpi_grid* grid, const gpi_array* coefs,
                      int32_t ncoefs, const gpi_pml* pml, int32_t npml_arrays,
                      uint64_t* model_out);
