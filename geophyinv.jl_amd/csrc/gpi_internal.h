@@ -119,6 +119,13 @@ struct PinnedStage {
     size_t cap = 0, off = 0;
 };
 
+// Fixed device region that a call carves its scratch buffers from instead of allocating
+// (gpi_forward_static): 256-byte aligned bump allocation, nothing is ever freed.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, off = 0;
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -128,6 +135,8 @@ struct Ctx {
     // once no queued work can reference them (the serving scheduler drains its streams first).
     std::vector<void*>* defer_free = nullptr;
     uint64_t* defer_bytes = nullptr;
+    // When set, Scratch buffers are carved from this region (no device allocation).
+    Arena* arena = nullptr;
     hipEvent_t ev[4] = {};
     std::vector<hipEvent_t> sample_ev;   // 4 events per sampled step
     uint64_t budget = 0, used = 0;

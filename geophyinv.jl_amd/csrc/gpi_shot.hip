@@ -391,20 +391,27 @@ int finish_timing(gpi_ctx* c, const Sampler& smp, int64_t nt, int64_t launches)
     return GPI_OK;
 }
 
-// Records of every receiver set into the caller's buffers (time fastest): transposed on
-// the device into one staging buffer, then copied; the stream has drained on return.
-// Returns GPI_ENONFINITE (after the copies) when a record holds Inf or NaN, as a shot
-// that violates the stability condition does (src/fdtd/stability.jl:8-60 only warns).
-int copy_records(gpi_ctx* c, const std::vector<DevPoints>& rec, int64_t nt, float* const* out)
+// Floats of the staging buffer copy_records transposes one receiver set into.
+size_t records_stage_floats(const std::vector<DevPoints>& rec, int64_t nt)
 {
     size_t most = 0;
     for (const DevPoints& D : rec) most = std::max(most, (size_t)nt * (size_t)std::max(D.npts, 0));
+    return most;
+}
+
+// Records of every receiver set into the caller's buffers (time fastest): transposed on
+// the device into one staging buffer (`tmp`, records_stage_floats, and the int `flag`, or
+// scratch buffers when null), then copied; the stream has drained on return.
+// Returns GPI_ENONFINITE (after the copies) when a record holds Inf or NaN, as a shot
+// that violates the stability condition does (src/fdtd/stability.jl:8-60 only warns).
+int copy_records(gpi_ctx* c, const std::vector<DevPoints>& rec, int64_t nt, float* const* out,
+                 float* tmp = nullptr, int* flag = nullptr)
+{
+    const size_t most = records_stage_floats(rec, nt);
     if (!most) return GPI_OK;
     Scratch S(c);
-    float* tmp = nullptr;
-    int* flag = nullptr;
-    int rc = S.alloc(&tmp, most);
-    if (!rc) rc = S.alloc(&flag, 1);
+    int rc = tmp ? GPI_OK : S.alloc(&tmp, most);
+    if (!rc && !flag) rc = S.alloc(&flag, 1);
     if (rc) return rc;
     int bad = -1;
     for (size_t q = 0; q < rec.size(); ++q) {
@@ -475,6 +482,25 @@ int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
         HIPTRY(c, hipMemsetAsync(illum, 0, sizeof(double) * (size_t)m.slab, c->stream));
     }
 
+    // staging of the copy-back, taken before the first step so that a call whose buffers
+    // do not fit fails before any work is queued: transposed records + non-finite flag,
+    // and one packed field (final fields, snapshots, FP64 illumination)
+    float* rec_stage = nullptr;
+    int* rec_flag = nullptr;
+    if (const size_t most = records_stage_floats(rec, nt))
+        if ((rc = S.alloc(&rec_stage, most)) || (rc = S.alloc(&rec_flag, 1))) return rc;
+    size_t pack_bytes = 0;
+    auto field_bytes = [&](int f, size_t elem) {
+        int64_t n3[3];
+        shape_of(g.ndim, g.order, g.physics, g.n, f, n3);
+        pack_bytes = std::max(pack_bytes, elem * (size_t)(n3[0] * n3[1] * n3[2]));
+    };
+    for (int q = 0; q < shot->nfinal; ++q) field_bytes(shot->final_fields[q], sizeof(float));
+    if (shot->nsnap > 0) field_bytes(shot->snap_field, sizeof(float));
+    if (illum) field_bytes(GPI_P, sizeof(double));
+    char* pack = nullptr;
+    if (pack_bytes && (rc = S.alloc(&pack, pack_bytes))) return rc;
+
     // reset wavefields and CPML memory (reset_w2!, types.jl:101-113)
     if ((rc = zero_state(c, m, st, c->stream))) return rc;
 
@@ -529,24 +555,24 @@ int run_shot(gpi_ctx* c, Model& m, const gpi_shot* shot, Store* save)
 
     // copy back (everything, also when the records hold non-finite values: that code is
     // returned last, with the message copy_records left)
-    const int rec_rc = copy_records(c, rec, nt, shot->records);
+    const int rec_rc = copy_records(c, rec, nt, shot->records, rec_stage, rec_flag);
     if (rec_rc && rec_rc != GPI_ENONFINITE) return rec_rc;
     const std::string rec_msg = c->err;
     for (int q = 0; q < shot->nfinal; ++q) {
         const int f = shot->final_fields[q];
         int64_t n3[3];
         shape_of(g.ndim, g.order, g.physics, g.n, f, n3);
-        if ((rc = download_pitched(c, m, shot->final_out[q], st.field[f], n3))) return rc;
+        if ((rc = download_pitched(c, m, shot->final_out[q], st.field[f], n3, pack))) return rc;
     }
     for (int q = 0; q < shot->nsnap; ++q) {
         int64_t n3[3];
         shape_of(g.ndim, g.order, g.physics, g.n, shot->snap_field, n3);
-        if ((rc = download_pitched(c, m, shot->snap_out[q], snap[q], n3))) return rc;
+        if ((rc = download_pitched(c, m, shot->snap_out[q], snap[q], n3, pack))) return rc;
     }
     if (illum) {
         int64_t n3[3];
         shape_of(g.ndim, g.order, g.physics, g.n, GPI_P, n3);
-        if ((rc = download_pitched(c, m, shot->illum, illum, n3))) return rc;
+        if ((rc = download_pitched(c, m, shot->illum, illum, n3, pack))) return rc;
     }
     if (rec_rc) return fail(c, rec_rc, rec_msg);
     return GPI_OK;
@@ -569,15 +595,21 @@ int check_shot(gpi_ctx* c, uint64_t model, const gpi_shot* shot, Model** out)
 
 }  // namespace
 
-extern "C" {
+namespace gpi {
 
-int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
+int forward_shot(gpi_ctx* c, uint64_t model, const gpi_shot* shot)
 {
     Model* m = nullptr;
     int rc = check_shot(c, model, shot, &m);
     if (rc) return rc;
     return run_shot(c, *m, shot, nullptr);
 }
+
+}  // namespace gpi
+
+extern "C" {
+
+int gpi_forward(gpi_ctx* c, uint64_t model, const gpi_shot* shot) { return forward_shot(c, model, shot); }
 
 int gpi_forward_save(gpi_ctx* c, uint64_t model, const gpi_shot* shot, uint64_t* store_out)
 {

@@ -92,6 +92,9 @@ int plan_sources(gpi_ctx* c, const Model& m, Scratch& S, int nsrc, const gpi_poi
                  int64_t nt, bool velocity_only, PassExtras& XV, PassExtras& XS, Sources& out);
 int plan_receivers(gpi_ctx* c, const Model& m, const State& st, Scratch& S, const gpi_shot* shot, PassExtras& XV,
                    PassExtras& XS, std::vector<DevPoints>& rec);
+// the body of gpi_forward: one supersource on a resident model; every scratch buffer is
+// taken before the first step (from the context's arena when it has one)
+int forward_shot(gpi_ctx* c, uint64_t model, const gpi_shot* shot);
 
 // model construction pieces (gpi_api.hip) and the device coefficient preparation
 // (medium.hip), shared by the C ABI and the serving scheduler
@@ -108,11 +111,18 @@ int prep_coefficients_2d(gpi_ctx* c, Model& m, const float* vp, const float* vs,
 int copy_h2d(gpi_ctx* c, void* dst, const void* src, size_t bytes);
 
 int upload_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3]);
-// pitched device slab -> compact host array of logical size n[3] (blocking)
-int download_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3]);
-int download_pitched(gpi_ctx* c, const Model& m, double* dst, const double* src, const int64_t n[3]);
+// pitched device slab -> compact host array of logical size n[3] (blocking); packed through
+// `stage` (device, n[0]*n[1]*n[2] elements) when given, else through a scratch buffer
+int download_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3],
+                     void* stage = nullptr);
+int download_pitched(gpi_ctx* c, const Model& m, double* dst, const double* src, const int64_t n[3],
+                     void* stage = nullptr);
+
+// carve `bytes` (256-byte aligned) out of the context's arena; GPI_ENOMEM when it is full
+int arena_alloc(gpi_ctx* c, void** p, size_t bytes);
 
 // Scratch device buffers of one call, freed on every exit path (after the stream drained).
+// On a context with an arena the buffers are carved from it and nothing is freed.
 struct Scratch {
     gpi_ctx* c;
     std::vector<void*> p;
@@ -129,7 +139,7 @@ struct Scratch {
     {
         void* q = nullptr;
         const size_t b = std::max<size_t>(sizeof(T) * count, 4);
-        int rc = dev_alloc(c, &p, &bytes, &q, b);
+        int rc = c->arena ? arena_alloc(c, &q, b) : dev_alloc(c, &p, &bytes, &q, b);
         if (rc) return rc;
         *out = (T*)q;
         return GPI_OK;

@@ -124,6 +124,19 @@ int dev_alloc(gpi_ctx* c, std::vector<void*>* owner, uint64_t* owner_bytes, void
     return GPI_OK;
 }
 
+int arena_alloc(gpi_ctx* c, void** p, size_t bytes)
+{
+    Arena& a = *c->arena;
+    const size_t n = (size_t)round_up((int64_t)std::max<size_t>(bytes, 1), 256);
+    if (n > a.cap - a.off)
+        return fail(c, GPI_ENOMEM, "the call's device buffers exceed the static scratch of " + std::to_string(a.cap) +
+                                       " bytes (" + std::to_string(a.off) + " carved, " + std::to_string(n) +
+                                       " more asked)");
+    *p = a.base + a.off;
+    a.off += n;
+    return GPI_OK;
+}
+
 int alloc_coef(gpi_ctx* c, Model& m, int id)
 {
     const bool shifted = m.g.ndim == 3 && (id == GPI_C_DTINVAVXIRHO || id == GPI_C_DTINVAVYIRHO ||
@@ -380,14 +393,14 @@ __global__ __launch_bounds__(256) void pack_pitched_kernel(const T* __restrict__
 
 // Packed on the device, then one copy into the caller's array (no host-side staging slab).
 template <typename T>
-static int download_packed(gpi_ctx* c, const Model& m, T* dst, const T* src, const int64_t n[3])
+static int download_packed(gpi_ctx* c, const Model& m, T* dst, const T* src, const int64_t n[3], void* stage)
 {
     if (n[0] < 1 || n[1] < 1 || n[2] < 1 || n[0] > m.pitch0 || n[1] > 65535 || n[2] > 65535)
         return fail(c, GPI_EINVAL, "download: bad field size");
     const size_t cnt = (size_t)(n[0] * n[1] * n[2]);
     Scratch S(c);
-    T* tmp = nullptr;
-    int rc = S.alloc(&tmp, cnt);
+    T* tmp = (T*)stage;
+    int rc = tmp ? GPI_OK : S.alloc(&tmp, cnt);
     if (rc) return rc;
     hipLaunchKernelGGL(pack_pitched_kernel<T>, dim3((unsigned)((n[0] + 255) / 256), (unsigned)n[1], (unsigned)n[2]),
                        dim3(256), 0, c->stream, src, tmp, (int)n[0], (int)n[1], m.pitch0, m.pitch1);
@@ -397,14 +410,14 @@ static int download_packed(gpi_ctx* c, const Model& m, T* dst, const T* src, con
     return GPI_OK;
 }
 
-int download_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3])
+int download_pitched(gpi_ctx* c, const Model& m, float* dst, const float* src, const int64_t n[3], void* stage)
 {
-    return download_packed(c, m, dst, src, n);
+    return download_packed(c, m, dst, src, n, stage);
 }
 
-int download_pitched(gpi_ctx* c, const Model& m, double* dst, const double* src, const int64_t n[3])
+int download_pitched(gpi_ctx* c, const Model& m, double* dst, const double* src, const int64_t n[3], void* stage)
 {
-    return download_packed(c, m, dst, src, n);
+    return download_packed(c, m, dst, src, n, stage);
 }
 
 }  // namespace gpi

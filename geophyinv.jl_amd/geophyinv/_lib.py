@@ -144,6 +144,7 @@ SIGNATURES = {
     "gpi_jobs_get_stats": (C.c_int, [C.c_void_p, C.POINTER(gpi_jobs_stats)]),
     "gpi_latency": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double)]),
     "gpi_forward": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gpi_shot)]),
+    "gpi_forward_static": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gpi_shot)]),
     "gpi_forward_save": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gpi_shot), C.POINTER(C.c_uint64)]),
     "gpi_adjoint": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(gpi_adjoint_shot), C.c_float,
                               C.POINTER(C.c_float), C.POINTER(C.c_float)]),

@@ -446,12 +446,17 @@ class SeisForwExpt:
         return ((_lib.gpi_points * max(1, len(pts)))(*pts), (C.POINTER(C.c_float) * max(1, len(wavs)))(*wavs),
                 len(pts))
 
-    def run_shot(self, iss: int, final_fields=(), save: bool = False, snaps: bool = True, into=None):
+    def run_shot(self, iss: int, final_fields=(), save: bool = False, snaps: bool = True, into=None,
+                 static: bool = False):
         """Propagate supersource ``iss`` on the GPU; returns (records dict, finals dict).
 
         ``save`` runs the :forward_save variant, which keeps the boundary strips and the
         final snapshots of this supersource resident for a later :adjoint run.  ``into``
-        (field -> FP32 Fortran-ordered nt x nr array) receives the records in place."""
+        (field -> FP32 Fortran-ordered nt x nr array) receives the records in place.
+        ``static`` runs the shot through ``gpi_forward_static``, which takes its per-shot
+        device buffers from the library's static scratch array instead of allocating."""
+        if static and save:
+            raise ValueError("the static-scratch entry point runs :forward shots only")
         self._ensure_model()
         ctx = self._ctx
         nt = len(self.tgrid)
@@ -514,6 +519,8 @@ class SeisForwExpt:
             else:
                 ctx.check(ctx.lib.gpi_forward_save(ctx.h, self._model, C.byref(shot), C.byref(h)))
             self._stores[iss] = h.value
+        elif static:
+            ctx.check(ctx.lib.gpi_forward_static(ctx.h, self._model, C.byref(shot)))
         else:
             ctx.check(ctx.lib.gpi_forward(ctx.h, self._model, C.byref(shot)))
         self.timing = ctx.timing()

@@ -210,6 +210,20 @@ int gpi_model_get_coef(gpi_ctx* ctx, uint64_t model, int32_t coef, float* out, i
  * final wavefields) to the host. */
 int gpi_forward(gpi_ctx* ctx, uint64_t model, const gpi_shot* shot);
 
+/* gpi_forward without device allocation: the same shot, same records, but every per-shot
+ * device buffer (source and receiver tables, device records, transposed records, the
+ * non-finite flag, snapshot/illumination/packing slabs) is carved out of one static
+ * device array of GPI_STATIC_SCRATCH_BYTES that each device holds until the process
+ * ends; no hipMalloc or hipFree is called.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  A shot whose buffers exceed the array fails with GPI_ENOMEM before any
+ * step runs.  Every call on a device shares the one array, so calls that overlap on one
+ * device (several contexts, several host threads) race; the call drains its stream
+ * before returning.  The model itself is created as for gpi_forward (gpi_model_create,
+ * which does allocate). */
+#define GPI_STATIC_SCRATCH_BYTES (64u << 20)
+int gpi_forward_static(gpi_ctx* ctx, uint64_t model, const gpi_shot* shot);
+
 /* ---- time reversal and the reference gradient ---------------------------------------- */
 /* One supersource in :forward_save mode (src/fdtd/propagate.jl:138-261 with
  * boundary_save! at :229 and the snapshots of :251-258, src/fdtd/boundary.jl:217-306):
