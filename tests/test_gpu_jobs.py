@@ -90,6 +90,83 @@ def test_jobs_equal_plain_calls(gpu_ctx):
         pa.free()
 
 
+def test_jobs_at_the_elastic_serving_scale(gpu_ctx):
+    """The 3-D elastic serving workload's problem at its smallest size (128^3 + CPML =
+    168^3, dipping interface, explosive source, surface patch recording vx, vy, vz): a
+    forward job and a gradient job in one queue; the forward records against the oracle
+    (relative L2 1e-5), the gradient job's directional derivatives along a bump between
+    the source and the dipping interface against fourth-order central differences of the
+    FP64 misfit (1e-3)."""
+    import oracle as O
+    from geophyinv import gallery
+    print("job queue at 168^3: elastic forward vs oracle, gradient vs finite differences", flush=True)
+    nt = 300
+    true, _, ageom, srcwav, tgrid, kw = gallery.elastic_fwi_3d(n=128, nt=nt, nrec_side=32)
+    # the starting model differs from the true one within reach of 300 steps (the dipping
+    # interface's reflection returns later): a smooth anomaly 20 cells down
+    inner = true.vp.m.shape
+    idx = np.meshgrid(*[np.arange(s) for s in inner], indexing="ij")
+    blob = np.exp(-sum((a - c) ** 2 for a, c in zip(idx, (20.0, 66.0, 60.0))) / (2.0 * 5.0 ** 2))
+    start = true.copy()
+    start.vp.m[...] *= (1 - 0.05 * blob).astype(np.float32)
+    start.vs.m[...] *= (1 - 0.04 * blob).astype(np.float32)
+    start.rho.m[...] *= (1 + 0.03 * blob).astype(np.float32)
+    fwd = G.SeisForwExpt(G.FdtdElastic(), medium=true, tgrid=tgrid, ageom=ageom, srcwav=srcwav, context=gpu_ctx,
+                         **kw)
+    pg = G.SeisForwExpt(G.FdtdElastic("forward_save", 2), medium=start, tgrid=tgrid, ageom=ageom, srcwav=srcwav,
+                        context=gpu_ctx, checkpoint=50, **kw)
+    po = G.SeisForwExpt(G.FdtdElastic(), medium=true, tgrid=tgrid, ageom=ageom, srcwav=srcwav, context=gpu_ctx,
+                        **kw)
+    G.update(po)
+    dobs = [d.copy() for d in po.data]
+    po.free()
+    # gradient! first: it sets the coefficients from the model vector (a log/exp round
+    # trip), which the gradient job then runs on as well
+    loss = G.L2DistLoss()
+    m = pg.model_vector()
+    g = np.zeros_like(m)
+    G.gradient(g, m, loss, dobs, pg)
+    plain = {k: v.copy() for k, v in pg.gradients.items()}
+    with JobServer(workers=2) as js:
+        tf = js.submit(fwd, 0)
+        tg = js.submit(pg, 0, dobs=dobs[0])
+        of = js.wait(tf, timeout=600.0)
+        og = js.wait(tg, timeout=600.0)
+    orc = O.from_experiment(fwd)
+    orc.run()
+    for q, f in enumerate(fwd.rfields):
+        r = of["records"][f]
+        e = float(np.linalg.norm((r - orc.records(q)).astype(np.float64)) /
+                  np.linalg.norm(orc.records(q).astype(np.float64)))
+        print(f"  forward {f}: rel L2 vs oracle {e:.2e}", flush=True)
+        assert np.abs(r).max() > 0 and e <= 1e-5, (f, e)
+    for k, v in og["gradients"].items():
+        assert np.abs(plain[k]).max() > 0, k
+        assert np.array_equal(v, plain[k]), (k, np.abs(v - plain[k]).max())
+    J = lambda x: G.lossvalue(x, loss, dobs, pg)
+    J0 = J(m)
+    n = m.size // 3
+    # a bump (radius 3 cells, cut off at 9) at the depth of the anomaly, off to one side:
+    # on the paths of its reflections.  Cells outside the cut-off keep their model values
+    # bit for bit (a Gaussian tail of 1e-20 would still flip the last bit of some FP32
+    # coefficients, whose effect on the misfit is noise, not derivative)
+    r2 = sum((a - c) ** 2 for a, c in zip(idx, (20.0, 45.0, 50.0)))
+    d0 = np.where(r2 <= 81.0, np.exp(-r2 / (2.0 * 3.0 ** 2)), 0.0).ravel(order="F")
+    for k, name in enumerate(("invlambda", "invmu", "rho")):
+        d = np.zeros_like(m)
+        d[k * n:(k + 1) * n] = d0
+        ad = float(np.dot(g.astype(np.float64), d))
+        # a step that moves the misfit by about 1e-3 of itself: above the noise of FP32
+        # wavefields in the misfit (about 1e-6 of it here), inside the linear range
+        e = min(max(1e-3 * J0 / abs(ad), 1e-3), 0.1)
+        fd = (8.0 * (J(m + e * d) - J(m - e * d)) - (J(m + 2 * e * d) - J(m - 2 * e * d))) / (12.0 * e)
+        rel = abs(ad / fd - 1)
+        print(f"  gradient {name}: J {J0:.4e} eps {e:.1e} fd {fd:.6e} adjoint {ad:.6e} rel {rel:.2e}", flush=True)
+        assert rel < 1e-3, (name, ad, fd)
+    fwd.free()
+    pg.free()
+
+
 def test_jobs_one_at_a_time_under_a_small_budget(gpu_ctx):
     fwd = [elastic3d(gpu_ctx, 50 + k, mode="forward") for k in range(3)]
     plain = [pa.run_shot(0)[0] for pa in fwd]

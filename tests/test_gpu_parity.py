@@ -104,6 +104,18 @@ def test_elastic_full_size_grid_short(gpu_ctx):
     compare(pa, ["tauxx", "vz"], "elastic 2081^2 x 40 steps")
 
 
+def test_elastic_drop_in_workload_full_length(gpu_ctx):
+    """The 2001^2 P-SV drop-in workload as the bench runs it: 2081^2 with CPML, all 4000
+    steps, its own 20 Hz explosive Ricker, 801 surface receivers (vx, vz), against the
+    oracle on the host (about 17 G cell-steps, some 25 s on 16 host threads)."""
+    med, ageom, srcwav, tgrid, kw = gallery.elastic_layers_2d()
+    assert len(tgrid) == 4000
+    pa = G.SeisForwExpt(G.FdtdElastic(), medium=med, tgrid=tgrid, ageom=ageom, srcwav=srcwav,
+                        context=gpu_ctx, **kw)
+    recs, _, _ = compare(pa, ["vx", "vz"], "elastic 2081^2 x 4000 steps")
+    assert np.abs(recs["vz"]).max() > 0
+
+
 def test_medium_update_resident(gpu_ctx):
     """update!(pa, medium) re-uploads coefficients in place; results follow the new medium."""
     med = layered_acoustic(nz=36, nx=40)

@@ -122,6 +122,31 @@ def test_server_elastic(gpu_ctx):
     assert st["completed"] == 5
 
 
+def test_served_stream_shots_against_the_oracle(gpu_ctx):
+    """Shots of the serving-stream generator (layered medium + anomaly, 20-cell CPML, p
+    source five cells down, p and vz surface receivers) with sides of 256-512 cells and
+    1000-1500 steps, served together; each shot's records against the oracle directly."""
+    import oracle as O
+    from geophyinv import gallery
+    stream = gallery.serving_stream(3, seed=3003, side=(256, 512), steps=(1000, 1500))
+    pas = [G.SeisForwExpt(G.FdtdAcoustic(), medium=s["medium"], tgrid=s["tgrid"], ageom=s["ageom"],
+                          srcwav=s["srcwav"], context=gpu_ctx, **s["kw"]) for s in stream]
+    print("served stream shots vs oracle:", [(s["side"], s["nt"]) for s in stream], flush=True)
+    with Server(0, 2, chunk_steps=32) as srv:
+        tickets = [srv.submit(pa, 0) for pa in pas]
+        got = [srv.wait(t, timeout=120.0) for t in tickets]
+        st = srv.stats()
+    assert st["completed"] == 3 and st["mean_batch"] > 1.5
+    for k, pa in enumerate(pas):
+        orc = O.from_experiment(pa)
+        orc.run()
+        for q, f in enumerate(pa.rfields):
+            e = rel_l2(got[k][f], orc.records(q))
+            print(f"  shot {k} ({stream[k]['side']}^2, {stream[k]['nt']} steps) {f}: rel L2 {e:.1e}", flush=True)
+            assert np.abs(got[k][f]).max() > 0 and e <= TOL, (k, f, e)
+        pa.free()
+
+
 def test_server_rejects(gpu_ctx):
     with Server(0, 2) as srv:
         pa = shot(gpu_ctx, 7, order=4)
