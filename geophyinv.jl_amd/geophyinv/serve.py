@@ -180,6 +180,7 @@ class JobServer:
             p.len = a.size
             p.a, p.b, p.kI = _lib.fptr(a), _lib.fptr(b), _lib.fptr(k)
             pmls.append(p)
+            keep += [a, b, k]   # the queue reads them when the job starts, after pa may be gone
         ckeep, arrs = pa.coef_arrays()
         keep += ckeep
         j = _lib.gpi_job()

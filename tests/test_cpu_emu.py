@@ -79,6 +79,42 @@ def test_shots_equal_the_oracle(built):
     assert r.returncode == 0 and "shots equal the oracle" in r.stdout, (r.returncode, r.stderr[-3000:])
 
 
+JOBS = r"""
+import gc, sys
+sys.path[:0] = [{pkg!r}, {tests!r}]
+from geophyinv import _lib
+_lib.load({lib!r})
+import numpy as np
+from geophyinv.serve import JobServer
+from helpers import experiment, layered_acoustic
+ctx = _lib.Context(0)
+mk = lambda k: experiment(layered_acoustic(nz=48 + 4 * k, nx=52, seed=k), nt=200, nr=9, npml=8, ctx=ctx, seed=k)
+plain = [mk(k).run_shot(0)[0] for k in range(3)]
+with JobServer(workers=2) as js:
+    tickets = []
+    for k in range(3):
+        pa = mk(k)
+        tickets.append(js.submit(pa, 0))
+        del pa          # the caller lets go of the experiment right after submitting
+        gc.collect()
+        junk = [np.full(200000, np.nan, np.float32) for _ in range(8)]   # reuse of freed host memory
+    outs = [js.wait(t)["records"] for t in tickets]
+for a, b in zip(plain, outs):
+    for f in a:
+        assert np.array_equal(a[f], b[f]), f
+print("jobs keep their inputs")
+"""
+
+
+def test_job_queue_keeps_its_inputs(built):
+    """Regression: a job reads its inputs (CPML vectors among them) when a worker starts it,
+    which may be after the caller has dropped the experiment; the host wrapper keeps every
+    array it hands to the queue alive until the job is waited for."""
+    code = JOBS.format(pkg=os.path.join(ROOT, "geophyinv.jl_amd"), tests=os.path.join(ROOT, "tests"), lib=built)
+    r = run([sys.executable, "-c", code])
+    assert r.returncode == 0 and "jobs keep their inputs" in r.stdout, (r.returncode, r.stderr[-3000:])
+
+
 def test_serving_stream_equals_plain(built):
     r = run([sys.executable, os.path.join(EMU, "serving_sample.py"), "--lib", built, "--shots", "6", "--side", "64",
              "160", "--steps", "600", "900", "--plain", "--time-scale", "100", "--chunk-steps", "16"])
