@@ -130,6 +130,8 @@ def run_plain(reqs, ctx, G, prebuilt=False):
         pa.free()
         lat.append(time.perf_counter() - t_arr)
         out.append(recs)
+        if len(reqs) >= 256 and (k + 1) % 64 == 0:
+            print(f"plain path: {k + 1}/{len(reqs)} shots", flush=True)
     span = time.perf_counter() - t0
     return span, np.array(lat), [_digest(r) for r in out]
 
@@ -187,12 +189,78 @@ def run_jobs_stream(reqs, G, workers, prebuilt=False):
             tickets.append((js.submit(pa, 0), t_arr))
             del pa
         lat, out = [], []
-        for t, t_arr in tickets:
+        for k, (t, t_arr) in enumerate(tickets):
             out.append(js.wait(t)["records"])
             lat.append(time.perf_counter() - t_arr)
+            if len(tickets) >= 256 and (k + 1) % 64 == 0:
+                print(f"job queue: {k + 1}/{len(tickets)} shots", flush=True)
         span = time.perf_counter() - t0
         st = js.stats()
     return span, np.array(lat), [_digest(r) for r in out], st
+
+
+def run_fwi_optimizer(args, G, _lib):
+    """Optimizer work per FWI iteration at the 3-D gradient's model size (400^3 = 64 M FP64
+    log-parameters), outside the misfit/gradient evaluations: L-BFGS (memory 10) with the
+    two-loop recursion on the device and a strong-Wolfe line search on the host, against
+    SciPy's L-BFGS-B (history and compact-form products on the host).  The objective is a
+    separable quadratic evaluated with numpy (its time is excluded from both figures)."""
+    import numpy as np
+    from scipy.optimize import minimize
+
+    from geophyinv.fwi import DeviceLBFGS, wolfe_search
+
+    n = args.opt_n
+    iters = 6
+    rng = np.random.Generator(np.random.PCG64(1))
+    h = rng.uniform(0.5, 50.0, n)
+    b = rng.standard_normal(n)
+    t_eval = [0.0]
+    nfev = [0]
+
+    def fg(x):
+        t0 = time.perf_counter()
+        hx = h * x
+        f = 0.5 * float(np.dot(hx, x)) - float(np.dot(b, x))
+        g = hx - b
+        t_eval[0] += time.perf_counter() - t0
+        nfev[0] += 1
+        return f, g
+
+    out = {}
+    fmin = -0.5 * float(np.dot(b, b / h))     # the minimum; progress = (f - fmin) / (f0 - fmin)
+    ctx = _lib.Context(0)
+    x = np.ones(n)
+    t_eval[0], nfev[0] = 0.0, 0
+    t0 = time.perf_counter()
+    f0, g = fg(x)
+    opt = DeviceLBFGS(ctx, n, 10)
+    f = f0
+    for _ in range(iters):
+        d = opt.direction(x, g)
+        a0 = 1.0 if opt.pairs else min(1.0, 1.0 / float(np.linalg.norm(g)))
+        a, f, g, _ = wolfe_search(fg, x, f, g, d, a0)
+        x = x + a * d
+    opt.close()
+    total = time.perf_counter() - t0
+    out["device_lbfgs"] = {"host_s_per_iteration": round((total - t_eval[0]) / iters, 3), "iterations": iters,
+                           "evaluations": nfev[0], "gap_ratio": (f - fmin) / (f0 - fmin)}
+    ctx.close()
+    x = np.ones(n)
+    t_eval[0], nfev[0] = 0.0, 0
+    t0 = time.perf_counter()
+    res = minimize(fg, x, jac=True, method="L-BFGS-B", options={"maxiter": iters, "gtol": 1e-25, "ftol": 0.0})
+    total = time.perf_counter() - t0
+    out["scipy_lbfgsb"] = {"host_s_per_iteration": round((total - t_eval[0]) / max(res.nit, 1), 3),
+                           "iterations": int(res.nit), "evaluations": nfev[0],
+                           "gap_ratio": (float(res.fun) - fmin) / (f0 - fmin)}
+    return {"metric": "optimizer host seconds per FWI iteration (64 M parameters)",
+            "value": out["device_lbfgs"]["host_s_per_iteration"], "unit": "s", "n_gpus": 1, "steps": iters,
+            "warmup": 0, "ms_per_step": out["device_lbfgs"]["host_s_per_iteration"] * 1e3,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"L-BFGS memory 10 on a separable quadratic of {n} FP64 parameters, {iters} "
+                                   "iterations; evaluation time excluded", "parameters": n},
+            **out}
 
 
 def run_serve_jobs(args, G, _lib):
@@ -605,13 +673,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt", type=int, default=0, help="time steps (default: the workload's own)")
     ap.add_argument("--workload", choices=["elastic2d", "acoustic3d", "elastic3d", "gradient3d", "gradient3d_elastic",
-                                           "serve", "serve3d", "serve_jobs"], default="elastic2d")
+                                           "serve", "serve3d", "serve_jobs", "fwi_optimizer"], default="elastic2d")
+    ap.add_argument("--opt-n", type=int, default=64_000_000, help="fwi_optimizer: parameters")
     ap.add_argument("--grad-fraction", type=float, default=0.25, help="serve3d: share of gradient requests")
     ap.add_argument("--workers", type=int, default=2, help="serve3d, serve_jobs: jobs in flight")
     ap.add_argument("--checkpoint", type=int, default=100, help="gradient3d: checkpoint interval")
     ap.add_argument("--fd-cells", type=int, default=0, help="gradient3d: finite-difference checks")
     ap.add_argument("--fd-target", type=float, default=1e-3, help="gradient3d: relative misfit change per FD step")
     ap.add_argument("--grid-n", type=int, default=400, help="gradient3d: cells per side of the medium")
+    ap.add_argument("--grid-2d", type=int, default=2001,
+                    help="elastic2d: cells per side of the medium (2001; 4001 puts each pass's working set in HBM)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     # Serving stream (the default 2-D elastic run appends it; --no-serve leaves it out).  The
@@ -656,6 +727,9 @@ def main():
     if args.workload == "serve3d":
         print(json.dumps(run_serve3d(args, G, _lib, gallery)), flush=True)
         return
+    if args.workload == "fwi_optimizer":
+        print(json.dumps(run_fwi_optimizer(args, G, _lib)), flush=True)
+        return
     if args.workload == "serve_jobs":
         if args.serve_rate is None:
             args.serve_rate = 200.0
@@ -677,7 +751,7 @@ def main():
         med, ageom, srcwav, tgrid, kw = gallery.acoustic_gradient_3d(nt=args.nt or 3000)
         attrib = G.FdtdAcoustic()
     else:
-        med, ageom, srcwav, tgrid, kw = gallery.elastic_layers_2d(nt=args.nt or 4000)
+        med, ageom, srcwav, tgrid, kw = gallery.elastic_layers_2d(n=args.grid_2d, nt=args.nt or 4000)
         attrib = G.FdtdElastic()
     ctx = _lib.Context(0)
     ctx.set_pass_sampling(64)
@@ -738,8 +812,9 @@ def main():
         trf = measured_traffic("fdtd3d_acoustic_440_traffic.md", "vel3d_acoustic")
     else:
         metric = "Gpt-updates/s (2-D elastic FDTD, FP32)"
-        workload = ("2-D elastic P-SV FdtdElastic, 2001x2001 at 2.5 m, 40-cell CPML (2081x2081), 8 layers + "
-                    "3% smoothed noise, explosive 20 Hz Ricker, 801 receivers (vx, vz), one shot per step")
+        workload = (f"2-D elastic P-SV FdtdElastic, {args.grid_2d}x{args.grid_2d} at 2.5 m, 40-cell CPML "
+                    f"({shape[0]}x{shape[1]}), 8 layers + 3% smoothed noise, explosive 20 Hz Ricker, 801 receivers "
+                    "(vx, vz), one shot per step")
         kname, other_name = "str2d_elastic<2> (stress pass)", "velocity_pass"
         trf = measured_traffic("fdtd2d_elastic_2081x2081_traffic.md", "str2d_elastic")
     line = {

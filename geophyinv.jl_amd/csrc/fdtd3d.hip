@@ -282,19 +282,12 @@ __global__ __launch_bounds__(512) void str3d_acoustic(StrParams3D P, PassExtras 
 // elastic
 // ---------------------------------------------------------------------------------------
 
+// One (z, y) point of plane k of the elastic velocity pass; bt is the injection bucket.
 template <int O>
-__global__ __launch_bounds__(512) void vel3d_elastic(VelParams3D P, PassExtras X)
+__device__ __forceinline__ void vel3d_elastic_point(const VelParams3D& P, const PassExtras& X, int i, int j, int k,
+                                                    int bt)
 {
     constexpr int H = O - 1;
-    const int b = P.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    const int nxw = P.nx - H;
-    if (b >= P.gx * P.gy * nxw) {
-        gather_block(X, b - P.gx * P.gy * nxw);
-        return;
-    }
-    Idx3 t;
-    const int bt = decode(b, P.gx, P.gy, nxw, P.xfast != 0, t);
-    const int i = t.i - H, j = t.j, k = t.k;   // rows shifted by h: the velocity rows start on 128-byte lines
     const int s1 = (int)P.s1, s2 = (int)P.s2;
     const float *txx = P.s[0], *tyy = P.s[1], *tzz = P.s[2], *txy = P.s[3], *txz = P.s[4], *tyz = P.s[5];
     const int o = (i + H) + s1 * (j + H) + s2 * (k + H);
@@ -385,18 +378,41 @@ __global__ __launch_bounds__(512) void vel3d_elastic(VelParams3D P, PassExtras X
     }
 }
 
+// The elastic passes march along x like the acoustic ones: workgroup (bx, by, chunk) runs
+// planes chunk*kc .. chunk*kc+kc-1 of its (z, y) tile one after another, so the planes
+// one plane's x-stencils share with the next are read again from this CU's caches while
+// they are still there, instead of by another workgroup much later (from HBM).  With
+// kc = 1 (one plane per workgroup) the xfast order is available as before.
 template <int O>
-__global__ __launch_bounds__(512) void str3d_elastic(StrParams3D P, PassExtras X)
+__global__ __launch_bounds__(512) void vel3d_elastic(VelParams3D P, PassExtras X)
 {
     constexpr int H = O - 1;
     const int b = P.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    if (b >= P.gx * P.gy * P.nx) {
-        gather_block(X, b - P.gx * P.gy * P.nx);
+    const int nxw = P.nx - H;
+    const int nch = (nxw + P.kc - 1) / P.kc;
+    if (b >= P.gx * P.gy * nch) {
+        gather_block(X, b - P.gx * P.gy * nch);
         return;
     }
-    Idx3 t;
-    const int bt = decode(b, P.gx, P.gy, P.nx, P.xfast != 0, t);
-    const int i = t.i, j = t.j, k = t.k;
+    if (P.kc == 1) {
+        Idx3 t;
+        const int bt = decode(b, P.gx, P.gy, nxw, P.xfast != 0, t);
+        vel3d_elastic_point<O>(P, X, t.i - H, t.j, t.k, bt);   // rows shifted by h
+        return;
+    }
+    const int bx = b % P.gx, r = b / P.gx, by = r % P.gy, ch = r / P.gy;
+    const int i = bx * 64 + (int)threadIdx.x - H;   // rows shifted by h: velocity rows on 128-byte lines
+    const int j = by * (int)blockDim.y + (int)threadIdx.y;
+    const int k0 = ch * P.kc, k1 = min(k0 + P.kc, nxw);
+    for (int k = k0; k < k1; ++k) vel3d_elastic_point<O>(P, X, i, j, k, bx + P.gx * (by + P.gy * k));
+}
+
+// One (z, y) point of plane k of the elastic stress pass; bt is the injection bucket.
+template <int O>
+__device__ __forceinline__ void str3d_elastic_point(const StrParams3D& P, const PassExtras& X, int i, int j, int k,
+                                                    int bt)
+{
+    constexpr int H = O - 1;
     const int s1 = (int)P.s1, s2 = (int)P.s2;
     const bool fs = (P.free_mask & GPI_FACE(0, 0)) != 0;
     const int o = i + s1 * j + s2 * k;
@@ -508,6 +524,28 @@ __global__ __launch_bounds__(512) void str3d_elastic(StrParams3D P, PassExtras X
     }
 }
 
+template <int O>
+__global__ __launch_bounds__(512) void str3d_elastic(StrParams3D P, PassExtras X)
+{
+    const int b = P.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int nch = (P.nx + P.kc - 1) / P.kc;
+    if (b >= P.gx * P.gy * nch) {
+        gather_block(X, b - P.gx * P.gy * nch);
+        return;
+    }
+    if (P.kc == 1) {
+        Idx3 t;
+        const int bt = decode(b, P.gx, P.gy, P.nx, P.xfast != 0, t);
+        str3d_elastic_point<O>(P, X, t.i, t.j, t.k, bt);
+        return;
+    }
+    const int bx = b % P.gx, r = b / P.gx, by = r % P.gy, ch = r / P.gy;
+    const int i = bx * 64 + (int)threadIdx.x;
+    const int j = by * (int)blockDim.y + (int)threadIdx.y;
+    const int k0 = ch * P.kc, k1 = min(k0 + P.kc, P.nx);
+    for (int k = k0; k < k1; ++k) str3d_elastic_point<O>(P, X, i, j, k, bx + P.gx * (by + P.gy * k));
+}
+
 // x-planes per workgroup of the acoustic passes: GPI_3D_MARCH planes (default 4, measured
 // on 440^3 against 2, 8 and 16; 1 gives one plane per workgroup), at most the axis length
 // XCD-aware workgroup order of the 3-D passes: GPI_XCD_REMAP3 (default 0: measured on
@@ -548,6 +586,19 @@ bool nt_stores()
 // Measured on 360^3 (order 2): plain order 24.9 Gpt/s, x-fastest 23.3, x-fastest with
 // the XCD-aware order 21.2; marching along x with the x-stencils in registers (as the
 // acoustic passes do) 16-18, its 166-256 VGPRs leaving 1-3 waves per SIMD against 7.
+// x-planes per workgroup of the elastic passes: GPI_3D_EMARCH (default 1 = one plane per
+// workgroup), at most the axis length
+int elastic_march_planes(int n)
+{
+    static int kc = -1;
+    if (kc < 0) {
+        const char* e = getenv("GPI_3D_EMARCH");
+        kc = e ? atoi(e) : 1;
+        if (kc < 1) kc = 1;
+    }
+    return std::min(kc, std::max(n, 1));
+}
+
 bool elastic_xfast()
 {
     static int e = -1;
@@ -565,7 +616,7 @@ void launch_vel3(const Model& m, hipStream_t s, VelParams3D P, const PassExtras&
     P.gx = (P.nz - H + vel3d_row_shift(m) + 63) / 64;
     const int ty = tile_rows_3d(m);
     P.gy = (P.ny - H + ty - 1) / ty;
-    P.kc = m.g.physics == 0 ? march_planes(P.nx - H) : 1;
+    P.kc = m.g.physics == 0 ? march_planes(P.nx - H) : elastic_march_planes(P.nx - H);
     P.remap = remap3();
     P.xfast = m.g.physics == 1 && elastic_xfast();
     dim3 grd(P.gx * P.gy * ((P.nx - H + P.kc - 1) / P.kc) + X.gather_blocks), blk(64, ty);
@@ -580,7 +631,7 @@ void launch_str3(const Model& m, hipStream_t s, StrParams3D P, const PassExtras&
     P.gx = (P.nz + 63) / 64;
     const int ty = tile_rows_3d(m);
     P.gy = (P.ny + ty - 1) / ty;
-    P.kc = m.g.physics == 0 ? march_planes(P.nx) : 1;
+    P.kc = m.g.physics == 0 ? march_planes(P.nx) : elastic_march_planes(P.nx);
     P.remap = remap3();
     P.xfast = m.g.physics == 1 && elastic_xfast();
     dim3 grd(P.gx * P.gy * ((P.nx + P.kc - 1) / P.kc) + X.gather_blocks), blk(64, ty);

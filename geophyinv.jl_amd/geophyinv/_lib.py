@@ -156,6 +156,13 @@ SIGNATURES = {
                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "gpi_adjoint_checkpoint_elastic": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(gpi_adjoint_shot),
                                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "gpi_interp_apply": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                   C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "gpi_lbfgs_create": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]),
+    "gpi_lbfgs_destroy": (None, [C.c_void_p]),
+    "gpi_lbfgs_reset": (C.c_int, [C.c_void_p]),
+    "gpi_lbfgs_direction": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     "gpi_last_timing": (C.c_int, [C.c_void_p, C.POINTER(gpi_timing)]),
     "gpi_set_pass_sampling": (C.c_int, [C.c_void_p, C.c_int32]),
 }

@@ -432,6 +432,30 @@ int gpi_jobs_submit(gpi_jobs* q, const gpi_job* job, uint64_t* ticket);
 int gpi_jobs_wait(gpi_jobs* q, uint64_t ticket, double timeout_s, int32_t* status);
 int gpi_jobs_get_stats(gpi_jobs* q, gpi_jobs_stats* out);
 
+/* ---- the FWI outer loop on the device ------------------------------------------------- */
+/* Separable linear map of apply_proj_matrix! (src/proj_mat.jl:208-226): y is x contracted
+ * with w[0] along axis 0, then w[1] along axis 1 (then w[2] along axis 2), where w[a] is the
+ * dense column-major FP32 (nin[a] x nout[a]) matrix of 1-D interpolation weights of
+ * get_proj_matrix (:176-200) or its transpose; FP64 sums, x of nin[0] x .. and y of
+ * nout[0] x .. elements (z fastest), host arrays.  The inversion-grid model to the
+ * modelling grid (P) and the modelling-grid gradient back to the inversion grid (P'). */
+int gpi_interp_apply(gpi_ctx* ctx, int32_t ndim, const int64_t nin[3], const int64_t nout[3],
+                     const float* const w[3], const double* x, double* y);
+
+/* L-BFGS direction with the pair history resident in HBM (the optimizer the reference
+ * drives from src/fwi/fwi.jl:66-75, Optim.jl LBFGS with memory 10).  Each call takes the
+ * current model m and gradient g (n doubles each, host); from the second call on it first
+ * folds the pair (m - m_prev, g - g_prev) into the history when s'y > 0 (at most `memory`
+ * pairs, oldest dropped), then returns d = -H g by the two-loop recursion with H0 = gamma I,
+ * gamma = s'y / y'y of the newest pair (d = -g while the history is empty), and the number
+ * of pairs used.  No reference counterpart for the split: the reference keeps the history
+ * on the host. */
+typedef struct gpi_lbfgs gpi_lbfgs;
+int gpi_lbfgs_create(gpi_ctx* ctx, int64_t n, int32_t memory, gpi_lbfgs** out);
+void gpi_lbfgs_destroy(gpi_lbfgs* lbfgs);
+int gpi_lbfgs_reset(gpi_lbfgs* lbfgs);   /* forget the history and the previous point */
+int gpi_lbfgs_direction(gpi_lbfgs* lbfgs, const double* m, const double* g, double* d, int32_t* pairs);
+
 /* Device timing of the last gpi_forward on this context. */
 int gpi_last_timing(const gpi_ctx* ctx, gpi_timing* out);
 

@@ -346,3 +346,34 @@ def oracle_records(medium, pa_like):
         out.append(G.Recs(a.nr, pa_like.tgrid, pa_like.rfields,
                           {f: o.records(q) for q, f in enumerate(pa_like.rfields)}))
     return out
+
+
+# ---------------------------------------------------------------------------------------
+# Host restatements the device FWI pieces are checked against
+# ---------------------------------------------------------------------------------------
+
+def apply_proj_host(m, mats):
+    """apply_proj_matrix! (src/proj_mat.jl:208-226) with numpy: m contracted with mats[0]
+    along axis 0, then mats[1] along axis 1 (...), FP64 sums, stored FP32."""
+    out = np.asarray(m, np.float64)
+    for ax, P in enumerate(mats):
+        out = np.moveaxis(np.tensordot(np.asarray(P, np.float64), out, axes=([0], [ax])), 0, ax)
+    return out.astype(np.float32)
+
+
+def lbfgs_two_loop(pairs, g):
+    """-H g of L-BFGS (Nocedal & Wright, Algorithm 7.4) for the (s, y) pairs, oldest first,
+    with H0 = gamma I, gamma = s'y / y'y of the newest pair (H0 = I without pairs)."""
+    q = np.array(g, np.float64)
+    alphas = []
+    for s, y in reversed(pairs):
+        a = float(np.dot(s, q)) / float(np.dot(s, y))
+        alphas.append(a)
+        q -= a * y
+    if pairs:
+        s, y = pairs[-1]
+        q *= float(np.dot(s, y)) / float(np.dot(y, y))
+    for (s, y), a in zip(pairs, reversed(alphas)):
+        b = float(np.dot(y, q)) / float(np.dot(s, y))
+        q += (a - b) * s
+    return -q

@@ -8,9 +8,9 @@ import numpy as np
 import pytest
 
 import geophyinv as G
-from geophyinv.fwi import apply_proj, interp_matrix
+from geophyinv.fwi import interp_matrix, wolfe_search
 from geophyinv.grid import Grid
-from helpers import fwi_experiment, layered_acoustic
+from helpers import apply_proj_host, lbfgs_two_loop
 
 
 def test_interp_matrix_partition_and_linear_reproduction():
@@ -41,20 +41,50 @@ def test_transposed_projection_is_the_adjoint():
     PT = [np.ascontiguousarray(p.T) for p in P]
     m = rng.standard_normal((6, 8))
     y = rng.standard_normal((30, 37))
-    lhs = np.sum(apply_proj(m, P).astype(np.float64) * y)
-    rhs = np.sum(m * apply_proj(y, PT).astype(np.float64))
+    lhs = np.sum(apply_proj_host(m, P).astype(np.float64) * y)
+    rhs = np.sum(m * apply_proj_host(y, PT).astype(np.float64))
     assert lhs == pytest.approx(rhs, rel=1e-5)
 
 
-def test_seisinvexpt_model_vector_round_trip():
-    med = layered_acoustic(nz=40, nx=52, seed=2)
-    pa = fwi_experiment(med, nss=1, nt=50)
-    paw = G.SeisInvExpt(pa, [G.Recs(a.nr, pa.tgrid, pa.rfields) for a in pa.ageom], [8, 10])
-    assert [g.n for g in paw.migrid] == [8, 10]
-    assert paw.migrid[0].first == pytest.approx(pa.medium.grid[0].first + 4 * pa.medium.grid[0].step)
-    m = paw.get_modelvector()
-    assert m.shape == (paw.nm,)
-    # log-parameters of a layered model are O(0.1) around the reference value
-    assert np.all(np.isfinite(m)) and np.abs(m).max() < 2.0
-    paw._to_full(m)
-    assert paw.mfull.shape == pa.model_vector().shape
+def test_two_loop_equals_the_dense_bfgs_inverse():
+    """The two-loop recursion the device L-BFGS is checked against equals -H g with H the
+    BFGS inverse Hessian built densely from H0 = gamma I through the same pairs
+    (H <- (I - rho s y') H (I - rho y s') + rho s s'), and with no pairs it is -g."""
+    rng = np.random.Generator(np.random.PCG64(7))
+    n = 12
+    A = rng.standard_normal((n, n))
+    A = A @ A.T + n * np.eye(n)                      # SPD: every pair has s'y > 0
+    pairs = []
+    for _ in range(5):
+        s = rng.standard_normal(n)
+        pairs.append((s, A @ s))
+    g = rng.standard_normal(n)
+    s_n, y_n = pairs[-1]
+    H = (s_n @ y_n) / (y_n @ y_n) * np.eye(n)
+    for s, y in pairs:
+        rho = 1.0 / (s @ y)
+        V = np.eye(n) - rho * np.outer(y, s)
+        H = V.T @ H @ V + rho * np.outer(s, s)
+    assert np.allclose(lbfgs_two_loop(pairs, g), -H @ g, rtol=1e-11, atol=1e-12)
+    assert np.array_equal(lbfgs_two_loop([], g), -g)
+    # a mistake in either loop's order shows: reversing the pairs changes the direction
+    assert not np.allclose(lbfgs_two_loop(pairs[::-1], g), -H @ g, rtol=1e-6)
+
+
+def test_wolfe_search_satisfies_the_strong_wolfe_conditions():
+    """The host line search of the device L-BFGS on the Rosenbrock function from several
+    points and initial steps: the step it returns meets sufficient decrease and the strong
+    curvature condition."""
+    def fg(x):
+        f = 100.0 * (x[1] - x[0] ** 2) ** 2 + (1 - x[0]) ** 2
+        g = np.array([-400.0 * x[0] * (x[1] - x[0] ** 2) - 2 * (1 - x[0]), 200.0 * (x[1] - x[0] ** 2)])
+        return f, g
+    for x0, a0 in (((-1.2, 1.0), 1.0), ((0.5, -0.3), 1e-3), ((2.0, 2.0), 10.0)):
+        x = np.array(x0)
+        f0, g0 = fg(x)
+        d = -g0
+        a, f, g, nfev = wolfe_search(fg, x, f0, g0, d, a0)
+        assert a is not None and nfev <= 20
+        assert f <= f0 + 1e-4 * a * (g0 @ d)
+        assert abs(g @ d) <= 0.9 * abs(g0 @ d)
+        assert f == pytest.approx(fg(x + a * d)[0])
