@@ -633,7 +633,11 @@ def fd_check_3d(args, pa, m, g, J, dobs, loss, mp, start, G, blocks=(0,)):
         bv[blk * nb:blk * nb + b_len] = b.ravel(order="F")
         return bv
 
-    cands = np.stack([rng.integers(lo, s - lo, 4 * args.fd_cells) for s in n], axis=1)
+    # depth from 20 cells (clear of the source five cells down and its bump window) to 150,
+    # where the surface acquisition's waves carry the misfit; y and x clear of the CPML
+    zlo, zhi = min(20, n[0] // 4), max(min(150, n[0] - lo), min(20, n[0] // 4) + 1)
+    cands = np.stack([rng.integers(zlo, zhi, 4 * args.fd_cells)] +
+                     [rng.integers(lo, s - lo, 4 * args.fd_cells) for s in n[1:]], axis=1)
     sens = []
     for c in cands:
         for k in blocks:

@@ -87,15 +87,16 @@ struct Idx3 {
     int i, j, k;
 };
 
-// Workgroup b -> point (i, j, k) of a 64 x 4 tile of plane k; returns the per-plane tile
-// id bx + gx*(by + gy*k) (the injection bucket).  With xfast the planes of one tile are
-// consecutive workgroups (nk planes), so that with the XCD-aware order the x-neighbour
-// planes a point's x-stencil reads are handled on the same XCD.
+// Workgroup b -> point (i, j, k) of a 64 x TY tile of plane k (workgroup chunk times
+// blockDim.z plus threadIdx.z); returns the per-plane tile id bx + gx*(by + gy*k) (the
+// injection bucket).  With xfast the chunks of one tile are consecutive workgroups (nk
+// chunks), so that with the XCD-aware order the x-neighbour planes a point's x-stencil
+// reads are handled on the same XCD.
 __device__ __forceinline__ int decode(int b, int gx, int gy, int nk, bool xfast, Idx3& p)
 {
-    int bx, by;
+    int bx, by, kc;
     if (xfast) {
-        p.k = b % nk;
+        kc = b % nk;
         const int t = b / nk;
         bx = t % gx;
         by = t / gx;
@@ -103,8 +104,9 @@ __device__ __forceinline__ int decode(int b, int gx, int gy, int nk, bool xfast,
         bx = b % gx;
         const int r = b / gx;
         by = r % gy;
-        p.k = r / gy;
+        kc = r / gy;
     }
+    p.k = kc * (int)blockDim.z + (int)threadIdx.z;
     p.i = bx * 64 + (int)threadIdx.x;
     p.j = by * (int)blockDim.y + (int)threadIdx.y;
     return bx + gx * (by + gy * p.k);
@@ -380,31 +382,28 @@ __device__ __forceinline__ void vel3d_elastic_point(const VelParams3D& P, const 
 
 // The elastic passes march along x like the acoustic ones: workgroup (bx, by, chunk) runs
 // planes chunk*kc .. chunk*kc+kc-1 of its (z, y) tile one after another, so the planes
-// one plane's x-stencils share with the next are read again from this CU's caches while
-// they are still there, instead of by another workgroup much later (from HBM).  With
-// kc = 1 (one plane per workgroup) the xfast order is available as before.
+// Elastic workgroups are 64 (z) x TY (y) x KZ (x) threads: the layer threadIdx.z of
+// workgroup chunk c updates plane c*KZ + threadIdx.z, so the planes whose x-stencils
+// share samples are updated by one workgroup at the same time and the shared samples are
+// served by its CU's caches rather than fetched again by another workgroup later (as a
+// loop over planes would do, but without the registers the compiler keeps live across
+// such a loop).  With KZ = 1 the xfast order is available as before.
 template <int O>
 __global__ __launch_bounds__(512) void vel3d_elastic(VelParams3D P, PassExtras X)
 {
     constexpr int H = O - 1;
     const int b = P.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int nxw = P.nx - H;
-    const int nch = (nxw + P.kc - 1) / P.kc;
+    const int kz = (int)blockDim.z;
+    const int nch = (nxw + kz - 1) / kz;
     if (b >= P.gx * P.gy * nch) {
-        gather_block(X, b - P.gx * P.gy * nch);
+        if (threadIdx.z == 0) gather_block(X, b - P.gx * P.gy * nch);
         return;
     }
-    if (P.kc == 1) {
-        Idx3 t;
-        const int bt = decode(b, P.gx, P.gy, nxw, P.xfast != 0, t);
-        vel3d_elastic_point<O>(P, X, t.i - H, t.j, t.k, bt);   // rows shifted by h
-        return;
-    }
-    const int bx = b % P.gx, r = b / P.gx, by = r % P.gy, ch = r / P.gy;
-    const int i = bx * 64 + (int)threadIdx.x - H;   // rows shifted by h: velocity rows on 128-byte lines
-    const int j = by * (int)blockDim.y + (int)threadIdx.y;
-    const int k0 = ch * P.kc, k1 = min(k0 + P.kc, nxw);
-    for (int k = k0; k < k1; ++k) vel3d_elastic_point<O>(P, X, i, j, k, bx + P.gx * (by + P.gy * k));
+    Idx3 t;
+    const int bt = decode(b, P.gx, P.gy, nch, P.xfast != 0, t);
+    if (t.k >= nxw) return;
+    vel3d_elastic_point<O>(P, X, t.i - H, t.j, t.k, bt);   // rows shifted by h
 }
 
 // One (z, y) point of plane k of the elastic stress pass; bt is the injection bucket.
@@ -528,22 +527,16 @@ template <int O>
 __global__ __launch_bounds__(512) void str3d_elastic(StrParams3D P, PassExtras X)
 {
     const int b = P.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    const int nch = (P.nx + P.kc - 1) / P.kc;
+    const int kz = (int)blockDim.z;
+    const int nch = (P.nx + kz - 1) / kz;
     if (b >= P.gx * P.gy * nch) {
-        gather_block(X, b - P.gx * P.gy * nch);
+        if (threadIdx.z == 0) gather_block(X, b - P.gx * P.gy * nch);
         return;
     }
-    if (P.kc == 1) {
-        Idx3 t;
-        const int bt = decode(b, P.gx, P.gy, P.nx, P.xfast != 0, t);
-        str3d_elastic_point<O>(P, X, t.i, t.j, t.k, bt);
-        return;
-    }
-    const int bx = b % P.gx, r = b / P.gx, by = r % P.gy, ch = r / P.gy;
-    const int i = bx * 64 + (int)threadIdx.x;
-    const int j = by * (int)blockDim.y + (int)threadIdx.y;
-    const int k0 = ch * P.kc, k1 = min(k0 + P.kc, P.nx);
-    for (int k = k0; k < k1; ++k) str3d_elastic_point<O>(P, X, i, j, k, bx + P.gx * (by + P.gy * k));
+    Idx3 t;
+    const int bt = decode(b, P.gx, P.gy, nch, P.xfast != 0, t);
+    if (t.k >= P.nx) return;
+    str3d_elastic_point<O>(P, X, t.i, t.j, t.k, bt);
 }
 
 // x-planes per workgroup of the acoustic passes: GPI_3D_MARCH planes (default 4, measured
@@ -586,17 +579,19 @@ bool nt_stores()
 // Measured on 360^3 (order 2): plain order 24.9 Gpt/s, x-fastest 23.3, x-fastest with
 // the XCD-aware order 21.2; marching along x with the x-stencils in registers (as the
 // acoustic passes do) 16-18, its 166-256 VGPRs leaving 1-3 waves per SIMD against 7.
-// x-planes per workgroup of the elastic passes: GPI_3D_EMARCH (default 1 = one plane per
-// workgroup), at most the axis length
-int elastic_march_planes(int n)
+// x-planes per elastic workgroup (its z dimension): GPI_3D_EKZ = 1, 2 or 4 (default 1),
+// reduced so that a workgroup has at most 512 threads
+int elastic_planes(int ty)
 {
-    static int kc = -1;
-    if (kc < 0) {
-        const char* e = getenv("GPI_3D_EMARCH");
-        kc = e ? atoi(e) : 1;
-        if (kc < 1) kc = 1;
+    static int kz = -1;
+    if (kz < 0) {
+        const char* e = getenv("GPI_3D_EKZ");
+        kz = e ? atoi(e) : 1;
+        if (kz != 2 && kz != 4) kz = 1;
     }
-    return std::min(kc, std::max(n, 1));
+    int k = kz;
+    while (k > 1 && 64 * ty * k > 512) k /= 2;
+    return k;
 }
 
 bool elastic_xfast()
@@ -616,10 +611,11 @@ void launch_vel3(const Model& m, hipStream_t s, VelParams3D P, const PassExtras&
     P.gx = (P.nz - H + vel3d_row_shift(m) + 63) / 64;
     const int ty = tile_rows_3d(m);
     P.gy = (P.ny - H + ty - 1) / ty;
-    P.kc = m.g.physics == 0 ? march_planes(P.nx - H) : elastic_march_planes(P.nx - H);
+    const int kz = m.g.physics == 0 ? 1 : elastic_planes(ty);
+    P.kc = m.g.physics == 0 ? march_planes(P.nx - H) : kz;
     P.remap = remap3();
     P.xfast = m.g.physics == 1 && elastic_xfast();
-    dim3 grd(P.gx * P.gy * ((P.nx - H + P.kc - 1) / P.kc) + X.gather_blocks), blk(64, ty);
+    dim3 grd(P.gx * P.gy * ((P.nx - H + P.kc - 1) / P.kc) + X.gather_blocks), blk(64, ty, kz);
     if (m.g.physics == 0 && nt_stores()) hipLaunchKernelGGL((vel3d_acoustic<O, true>), grd, blk, 0, s, P, X);
     else if (m.g.physics == 0) hipLaunchKernelGGL((vel3d_acoustic<O, false>), grd, blk, 0, s, P, X);
     else hipLaunchKernelGGL(vel3d_elastic<O>, grd, blk, 0, s, P, X);
@@ -631,10 +627,11 @@ void launch_str3(const Model& m, hipStream_t s, StrParams3D P, const PassExtras&
     P.gx = (P.nz + 63) / 64;
     const int ty = tile_rows_3d(m);
     P.gy = (P.ny + ty - 1) / ty;
-    P.kc = m.g.physics == 0 ? march_planes(P.nx) : elastic_march_planes(P.nx);
+    const int kz = m.g.physics == 0 ? 1 : elastic_planes(ty);
+    P.kc = m.g.physics == 0 ? march_planes(P.nx) : kz;
     P.remap = remap3();
     P.xfast = m.g.physics == 1 && elastic_xfast();
-    dim3 grd(P.gx * P.gy * ((P.nx + P.kc - 1) / P.kc) + X.gather_blocks), blk(64, ty);
+    dim3 grd(P.gx * P.gy * ((P.nx + P.kc - 1) / P.kc) + X.gather_blocks), blk(64, ty, kz);
     if (m.g.physics == 0 && nt_stores()) hipLaunchKernelGGL((str3d_acoustic<O, true>), grd, blk, 0, s, P, X);
     else if (m.g.physics == 0) hipLaunchKernelGGL((str3d_acoustic<O, false>), grd, blk, 0, s, P, X);
     else hipLaunchKernelGGL(str3d_elastic<O>, grd, blk, 0, s, P, X);
@@ -654,7 +651,7 @@ int tile_rows_3d(const Model& m)
     static int tye = -1;
     if (tye < 0) {
         const char* e = getenv("GPI_3D_TYE");
-        tye = e && atoi(e) == 8 ? 8 : 4;
+        tye = e && (atoi(e) == 8 || atoi(e) == 2) ? atoi(e) : 4;
     }
     return m.g.physics == 0 ? ty : tye;
 }
