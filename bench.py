@@ -117,7 +117,10 @@ def run_plain(reqs, ctx, G, prebuilt=False):
     the stream starts; the coefficients are still derived and uploaded per shot."""
     import numpy as np
 
-    pas = [_experiment(G, rq, context=ctx, coefficients="host") for rq in reqs] if prebuilt else None
+    if isinstance(prebuilt, list):   # experiments made by the caller
+        pas, prebuilt = list(prebuilt), True
+    else:
+        pas = [_experiment(G, rq, context=ctx, coefficients="host") for rq in reqs] if prebuilt else None
     t0 = time.perf_counter()
     lat, out = [], []
     for k, rq in enumerate(reqs):
@@ -174,7 +177,10 @@ def run_jobs_stream(reqs, G, workers, prebuilt=False):
     import numpy as np
     from geophyinv.serve import JobServer
 
-    pas = [_experiment(G, rq, coefficients="host") for rq in reqs] if prebuilt else None
+    if isinstance(prebuilt, list):   # experiments made by the caller
+        pas, prebuilt = list(prebuilt), True
+    else:
+        pas = [_experiment(G, rq, coefficients="host") for rq in reqs] if prebuilt else None
     with JobServer(workers=workers) as js:
         t0 = time.perf_counter()
         tickets = []
@@ -280,10 +286,13 @@ def run_serve_jobs(args, G, _lib):
     cells = float(sum((r["side"] + 2 * r["kw"]["npml"]) ** 2 * r["nt"] for r in reqs))
     print(f"serve_jobs: {nshots} shots, {cells / 1e12:.2f} T cell-steps", flush=True)
     ctx = _lib.Context(0)
-    p_span, p_lat, p_sums = run_plain(reqs, ctx, G, not args.serve_host_prep)
-    ctx.close()
+    # with host preparation before the stream, both paths take the same prepared experiments
+    prep = False if args.serve_host_prep else [_experiment(G, rq, context=ctx, coefficients="host") for rq in reqs]
+    p_span, p_lat, p_sums = run_plain(reqs, ctx, G, prep)
     print(f"serve_jobs: plain path {p_span:.1f} s", flush=True)
-    j_span, j_lat, j_sums, st = run_jobs_stream(reqs, G, args.workers, not args.serve_host_prep)
+    j_span, j_lat, j_sums, st = run_jobs_stream(reqs, G, args.workers, prep)
+    del prep
+    ctx.close()
     print(f"serve_jobs: job queue {j_span:.1f} s", flush=True)
     same = sum(1 for a, b in zip(p_sums, j_sums) if a == b)
     pct = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 1)
@@ -535,6 +544,7 @@ def run_serve3d(args, G, _lib, gallery):
             mk = lambda true=true, tgrid=tgrid, ageom=ageom, srcwav=srcwav, kw=kw: G.SeisForwExpt(
                 G.FdtdElastic(), medium=true, tgrid=tgrid, ageom=ageom, srcwav=srcwav, context=ctx, **kw)
         shots.append(dict(n=n + 2 * kw["npml"], nt=nt, grad=grad, dobs=dobs, mk=mk))
+        print(f"serve3d: shot {k + 1}/{ns} made ({n}^3, {nt} steps{', gradient' if grad else ''})", flush=True)
     gen_s = time.perf_counter() - t0
     loss = G.L2DistLoss()
     # plain path
@@ -550,6 +560,7 @@ def run_serve3d(args, G, _lib, gallery):
             out_p.append(pa.run_shot(0)[0])
         pa.free()
         lat_p.append(time.perf_counter() - t0)
+        print(f"serve3d plain path: {len(lat_p)}/{ns} shots at {lat_p[-1]:.1f} s", flush=True)
     span_p = time.perf_counter() - t0
     # job queue: every shot submitted at t = 0 (from the same model state as the plain run)
     pas = [sh["mk"]() for sh in shots]
@@ -562,6 +573,7 @@ def run_serve3d(args, G, _lib, gallery):
         out_s = []
         for t in tickets:
             out_s.append(js.wait(t))
+            print(f"serve3d job queue: {len(out_s)}/{ns} shots at {time.perf_counter() - t0:.1f} s", flush=True)
         span_s = time.perf_counter() - t0
         st = js.stats()
     same = 0
@@ -694,6 +706,8 @@ def main():
     # the host before the stream: a 48-shot sample of the full 256..2048-cell stream faulted
     # the GPU once and the cause has not been shown, so sizes beyond 512 need the flags.
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="default line: leave out the 512-shot serving stream through the job queue")
     ap.add_argument("--serve-shots", type=int, default=None, help="serving: shots (128; serve3d 16)")
     ap.add_argument("--max-shots", type=int, default=64)
     ap.add_argument("--chunk-steps", type=int, default=32)
@@ -858,6 +872,18 @@ def main():
         line["serving"] = serving_section(args.serve_shots or 128, opts, G, _lib, args.serve_side or [128, 512],
                                           args.serve_steps or [500, 1500], args.serve_direct,
                                           args.serve_rate or 1e6, not args.serve_host_prep)
+    if not args.no_serve and not three and not args.no_stream:
+        # the serving-stream workload at its own sizes (512 shots of 256-2048 cells, 1000-6000
+        # steps, 200/s), through the job queue against the plain path
+        js = argparse.Namespace(**vars(args))
+        js.serve_shots, js.serve_side, js.serve_steps, js.serve_rate, js.workers = 512, [256, 2048], [1000, 6000], \
+            200.0, 4
+        r = run_serve_jobs(js, G, _lib)
+        line["serving_stream"] = {"workload": r["config"]["workload"], "workers": 4, "path": "job queue",
+                                  "arrivals_span_s": r["arrivals_span_s"], "gpt_updates": r["gpt_updates"],
+                                  "plain": r["plain"], "jobs": r["jobs"],
+                                  "speedup_shots_per_s": r["speedup_shots_per_s"],
+                                  "records_identical": r["records_identical"]}
     print(json.dumps(line), flush=True)
 
 
