@@ -10,6 +10,13 @@ roofline of the dominant kernel (the stress pass) is measured live with HIP even
 library's stream; bytes per launch are the algorithmic bytes of that pass.  The CPU baseline
 is the oracle (a C restatement of the reference's unfused pass structure, OpenMP) timed on
 a bounded number of time steps of the same workload on the host cores.
+
+The default line also carries two serving measurements: ``serving`` (128 2-D acoustic
+shots of at most 512 cells per side through the batched server and through plain calls)
+and ``serving_stream`` (the 512-shot serving stream at its own sizes, 256-2048 cells per
+side, through the job queue and through plain calls); both compare the records of the two
+paths.  Other workloads: ``--workload acoustic3d | elastic3d | gradient3d |
+gradient3d_elastic | serve | serve_jobs | serve3d | fwi_optimizer``.
 """
 import argparse
 import json

@@ -61,8 +61,10 @@ struct StrParams3D {
 // allocation for the same reason.  Returns the shift.
 int vel3d_row_shift(const Model& m);
 // y rows per workgroup of the 3-D passes: acoustic GPI_3D_TY (4 or 8, default 4: on 440^3
-// the 64 x 8 tile ran 9 % slower), elastic GPI_3D_TYE (4 or 8, default 4: on 360^3 the
-// 64 x 8 tile ran 12 % slower)
+// the 64 x 8 tile ran 9 % slower), elastic GPI_3D_TYE (2, 4 or 8, default 4: on 360^3 the
+// 64 x 8 tile ran 12 % slower).  The elastic passes also take GPI_3D_EKZ x-planes per
+// workgroup (1, 2 or 4, default 1, at most 512 threads): 64 x 2 x 2 fetches 20 % fewer
+// bytes than 64 x 4 x 1 at the same speed, the re-fetched planes being cache hits.
 int tile_rows_3d(const Model& m);
 int velocity_blocks_3d(const Model& m, int* gx, int* gy);
 int stress_blocks_3d(const Model& m, int* gx, int* gy);
