@@ -183,6 +183,70 @@ __device__ __forceinline__ void str_ac_point(const StrParams2D& P, const PassExt
     P.s0[o] = p;
 }
 
+// Elastic updates from their operands (the point functions below load them).
+template <int O>
+__device__ __forceinline__ float vel_el_vx(const VelParams2D& P, int i, int j, const float (&a1)[O],
+                                           const float (&a2)[O], float vxo, float cxo)
+{
+    // vx[inn] -= dtinvavxirho*(dtauxxdx + dtauxzdz)
+    float d1 = stencil<O>(a1) * P.dxI;               // D_xi(tauxx)
+    d1 = cpml_x(d1, P.qx0, i, j, P.npml, P.pml_mask);
+    float d2 = stencil<O>(a2) * P.dzI;               // D_za(tauxz)
+    d2 = cpml_z(d2, P.qz1, i, j, P.npml, P.pml_mask);
+    return vxo - cxo * (d1 + d2);
+}
+
+template <int O>
+__device__ __forceinline__ float vel_el_vz(const VelParams2D& P, int i, int j, const float (&b1)[O],
+                                           const float (&b2)[O], float vzo, float czo)
+{
+    // vz[inn] -= dtinvavzirho*(dtauxzdx + dtauzzdz)
+    float d1 = stencil<O>(b1) * P.dxI;               // D_xa(tauxz)
+    d1 = cpml_x(d1, P.qx1, i, j, P.npml, P.pml_mask);
+    float d2 = stencil<O>(b2) * P.dzI;               // D_zi(tauzz)
+    d2 = cpml_z(d2, P.qz0, i, j, P.npml, P.pml_mask);
+    return vzo - czo * (d1 + d2);
+}
+
+template <int O>
+__device__ __forceinline__ void str_el_normal(const StrParams2D& P, int i, int j, const float (&ax)[O],
+                                              const float (&az)[O], float s0o, float s1o, float m, float l,
+                                              float& txx, float& tzz)
+{
+    float dvx = stencil<O>(ax) * P.dxI;              // D_xa(vx)
+    dvx = cpml_x(dvx, P.qx0, i, j, P.npml, P.pml_mask);
+    float dvz = stencil<O>(az) * P.dzI;              // D_za(vz)
+    dvz = cpml_z(dvz, P.qz0, i, j, P.npml, P.pml_mask);
+    txx = (s0o - (m * dvx)) - (l * dvz);
+    tzz = (s1o - (m * dvz)) - (l * dvx);
+}
+
+// tauxx += pv*dtM and tauzz += pv*dtM per source field (source.jl:79-97)
+__device__ __forceinline__ void inject_normal2(const PassExtras& X, int64_t it, int ninj, int bk, int64_t o, float m,
+                                               float& txx, float& tzz)
+{
+    for (int q = 0; q < ninj; ++q) {
+        float acc;
+        int t;
+        if (X.inj[q].kind == INJ_NORMAL2 && injection_sum(X.inj[q], bk, o, it, acc, t)) {
+            txx = txx + (acc * m);
+            tzz = tzz + (acc * m);
+        }
+    }
+}
+
+template <int O>
+__device__ __forceinline__ float str_el_shear(const StrParams2D& P, int i, int j, const float (&bz)[O],
+                                              const float (&bx)[O], float s2o, float c2o)
+{
+    // tauxz -= dtavmu*(dvxdz + dvzdx)
+    float a = stencil<O>(bz) * P.dzI;                // D_zi(vx)
+    a = cpml_z(a, P.qz1, i, j, P.npml, P.pml_mask);
+    float c = stencil<O>(bx) * P.dxI;                // D_xi(vz)
+    c = cpml_x(c, P.qx1, i, j, P.npml, P.pml_mask);
+    return s2o - c2o * (a + c);
+}
+
 template <int O>
 __device__ __forceinline__ void vel_el_point(const VelParams2D& P, const PassExtras& X, int64_t it, int ninj, int i,
                                              int j)
@@ -210,25 +274,15 @@ __device__ __forceinline__ void vel_el_point(const VelParams2D& P, const PassExt
     }
     const float cxo = dt_over_av2(P.dt, rx0, rx1);   // dtinvavxirho
     const float czo = dt_over_av2(P.dt, rz0, rz1);   // dtinvavzirho
-    // vx[inn] -= dtinvavxirho*(dtauxxdx + dtauxzdz)
     if (ux) {
-        float d1 = stencil<O>(a1) * P.dxI;           // D_xi(tauxx)
-        d1 = cpml_x(d1, P.qx0, i, j, P.npml, P.pml_mask);
-        float d2 = stencil<O>(a2) * P.dzI;           // D_za(tauxz)
-        d2 = cpml_z(d2, P.qz1, i, j, P.npml, P.pml_mask);
-        float nv = vxo - cxo * (d1 + d2);
+        float nv = vel_el_vx<O>(P, i, j, a1, a2, vxo, cxo);
         if ((P.rigid_mask & GPI_FACE(2, 0)) && j == 0) P.vx[(i + H)] = -nv;
         if ((P.rigid_mask & GPI_FACE(2, 1)) && j == P.nx - H - 1) P.vx[(i + H) + ld * (P.nx + H - 1)] = -nv;
         if (ninj) nv = inject_velocity(X, it, INJ_VX, bk, o, nv, P.rho, P.dt);
         P.vx[o] = nv;
     }
-    // vz[inn] -= dtinvavzirho*(dtauxzdx + dtauzzdz)
     if (uz) {
-        float d1 = stencil<O>(b1) * P.dxI;           // D_xa(tauxz)
-        d1 = cpml_x(d1, P.qx1, i, j, P.npml, P.pml_mask);
-        float d2 = stencil<O>(b2) * P.dzI;           // D_zi(tauzz)
-        d2 = cpml_z(d2, P.qz0, i, j, P.npml, P.pml_mask);
-        float nv = vzo - czo * (d1 + d2);
+        float nv = vel_el_vz<O>(P, i, j, b1, b2, vzo, czo);
         if ((P.rigid_mask & GPI_FACE(0, 0)) && i == 0) P.vz[ld * (j + H)] = -nv;
         if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1) P.vz[(P.nz + H - 1) + ld * (j + H)] = -nv;
         if (ninj) nv = inject_velocity(X, it, INJ_VZ, bk, o, nv, P.rho, P.dt);
@@ -269,34 +323,16 @@ __device__ __forceinline__ void str_el_point(const StrParams2D& P, const PassExt
     const float l = (1.0f / il) * P.dt;                       // dtlambda
     const float c2o = dt_over_av4(P.dt, im, m10, m01, m11);   // dtavmu
     if (un) {
-        float dvx = stencil<O>(ax) * P.dxI;          // D_xa(vx)
-        dvx = cpml_x(dvx, P.qx0, i, j, P.npml, P.pml_mask);
-        float dvz = stencil<O>(az) * P.dzI;          // D_za(vz)
-        dvz = cpml_z(dvz, P.qz0, i, j, P.npml, P.pml_mask);
-        float txx = (s0o - (m * dvx)) - (l * dvz);
-        float tzz = (s1o - (m * dvz)) - (l * dvx);
+        float txx, tzz;
+        str_el_normal<O>(P, i, j, ax, az, s0o, s1o, m, l, txx, tzz);
         // free surface at zmin, before the sources: tauzz[1,:] = -tauzz[2,:]
         if (fs && i == 1) P.s1[ld * j] = -tzz;
-        // tauxx += pv*dtM and tauzz += pv*dtM per source field (source.jl:79-97)
-        const int bk = (i >> 6) + P.gx * j;
-        for (int q = 0; q < ninj; ++q) {
-            float acc;
-            int t;
-            if (X.inj[q].kind == INJ_NORMAL2 && injection_sum(X.inj[q], bk, o, it, acc, t)) {
-                txx = txx + (acc * m);
-                tzz = tzz + (acc * m);
-            }
-        }
+        if (ninj) inject_normal2(X, it, ninj, (i >> 6) + P.gx * j, o, m, txx, tzz);
         P.s0[o] = txx;
         if (!(fs && i == 0)) P.s1[o] = tzz;
     }
-    // tauxz -= dtavmu*(dvxdz + dvzdx)
     if (us) {
-        float a = stencil<O>(bz) * P.dzI;            // D_zi(vx)
-        a = cpml_z(a, P.qz1, i, j, P.npml, P.pml_mask);
-        float c = stencil<O>(bx) * P.dxI;            // D_xi(vz)
-        c = cpml_x(c, P.qx1, i, j, P.npml, P.pml_mask);
-        const float t = s2o - c2o * (a + c);
+        const float t = str_el_shear<O>(P, i, j, bz, bx, s2o, c2o);
         P.s2[o] = (fs && i == 0) ? 0.0f : t;         // tauxz[1,:] = 0
     }
 }

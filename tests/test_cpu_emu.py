@@ -79,6 +79,51 @@ def test_shots_equal_the_oracle(built):
     assert r.returncode == 0 and "shots equal the oracle" in r.stdout, (r.returncode, r.stderr[-3000:])
 
 
+ELASTIC = r"""
+import sys
+sys.path[:0] = [{pkg!r}, {orc!r}, {tests!r}]
+from geophyinv import _lib
+_lib.load({lib!r})
+import numpy as np
+import oracle as O
+from helpers import experiment, layered_elastic, rel_l2
+ctx = _lib.Context(0)
+FIN = ["tauxx", "tauzz", "tauxz", "vx", "vz"]
+cases = []
+for order in (2, 4, 6, 8):
+    # nz + npml padding not a multiple of 4 (partial four-row groups at both ends of a column)
+    cases.append(experiment(layered_elastic(nz=43 + order, nx=30), nt=150, sfields=("tauxx",), rfields=("vx", "vz"),
+                            ns=2, nr=7, order=order, npml=7, ctx=ctx, seed=40 + order))
+    cases.append(experiment(layered_elastic(nz=41 + order, nx=28), nt=150, sfields=("vz", "vx", "tauzz"),
+                            rfields=("vx", "vz"), ns=3, nr=6, order=order, npml=6, pml_faces=["zmax", "xmin", "xmax"],
+                            stressfree_faces=["zmin"], ctx=ctx, seed=50 + order, src_depth=6.0))
+# a column longer than 256 rows, sources near both ends
+cases.append(experiment(layered_elastic(nz=291, nx=18), nt=90, sfields=("vz", "tauxx"), rfields=("vx", "vz"), ns=2,
+                        nr=5, order=2, npml=5, ctx=ctx, seed=61))
+for k, pa in enumerate(cases):
+    orc = O.from_experiment(pa)
+    orc.run()
+    recs, fin = pa.run_shot(0, final_fields=FIN)
+    for q, f in enumerate(pa.rfields):
+        assert rel_l2(recs[f], orc.records(q)) == 0.0, (k, f, rel_l2(recs[f], orc.records(q)))
+    for f in FIN:
+        assert np.abs(fin[f]).max() > 0, (k, f)
+        assert np.array_equal(fin[f], orc.field(f)), (k, f, rel_l2(fin[f], orc.field(f)))
+print("elastic cases equal the oracle", len(cases))
+"""
+
+
+def test_elastic_orders_equal_the_oracle(built):
+    """The 2-D elastic kernels against the oracle through the CPU build: orders 2-8, free
+    surface with velocity and stress sources beside it, rigid faces, padded columns whose
+    length is not a multiple of 4, a column longer than one 256-row workgroup.  Records and
+    every final wavefield (ghost cells included) equal bit for bit."""
+    code = ELASTIC.format(pkg=os.path.join(ROOT, "geophyinv.jl_amd"), orc=os.path.join(ROOT, "oracle"),
+                       tests=os.path.join(ROOT, "tests"), lib=built)
+    r = run([sys.executable, "-c", code], timeout=900)
+    assert r.returncode == 0 and "elastic cases equal the oracle" in r.stdout, (r.returncode, r.stderr[-3000:])
+
+
 JOBS = r"""
 import gc, sys
 sys.path[:0] = [{pkg!r}, {tests!r}]
