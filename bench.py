@@ -16,7 +16,7 @@ shots of at most 512 cells per side through the batched server and through plain
 and ``serving_stream`` (the 512-shot serving stream at its own sizes, 256-2048 cells per
 side, through the job queue and through plain calls); both compare the records of the two
 paths.  Other workloads: ``--workload acoustic3d | elastic3d | gradient3d |
-gradient3d_elastic | serve | serve_jobs | serve3d | fwi_optimizer``.
+gradient3d_elastic | serve | serve_jobs | serve3d | fwi_optimizer | source_estimation``.
 """
 import argparse
 import json
@@ -274,6 +274,59 @@ def run_fwi_optimizer(args, G, _lib):
             "config": {"workload": f"L-BFGS memory 10 on a separable quadratic of {n} FP64 parameters, {iters} "
                                    "iterations; evaluation time excluded", "parameters": n},
             **out}
+
+
+def run_source_estimation(args, G, _lib):
+    """Source-signature estimation (update!(paw, pac)) for one supersource of the 2001^2
+    elastic workload's shape: 4000 samples x 2403 traces (801 receivers x vx, vz, p),
+    synthetic band-limited records convolved with a planted filter plus noise; LSMR with at
+    most 100 iterations.  Device: gpi_conv_lsmr (batched hipFFT, FP64).  Host: SciPy's LSMR
+    on conv.operator (NumPy FFTs), timed for a bounded number of iterations and reported
+    per iteration beside the device's."""
+    import numpy as np
+    from scipy.sparse.linalg import lsmr
+
+    from geophyinv import conv
+
+    nt, ntr = 4000, 2403
+    rng = np.random.Generator(np.random.PCG64(9))
+    t = np.arange(61)
+    pulse = np.exp(-0.5 * ((t - 30) / 6.0) ** 2) * np.cos(0.3 * (t - 30))
+    g = np.asfortranarray(np.stack([np.convolve(rng.standard_normal(nt), pulse)[:nt] for _ in range(ntr)], axis=1))
+    sl = (nt - 1) - (nt - 1) // 2
+    w = np.zeros(nt)
+    w[sl + 12], w[sl + 13], w[sl + 20] = 1.3, -0.5, 0.2
+    pa = conv.Pconv(dsize=(nt, ntr), ssize=(nt,), gsize=(nt, ntr), g=g, s=w)
+    d = np.asfortranarray(pa.mod_D() + 1e-3 * rng.standard_normal((nt, ntr)))
+    ctx = _lib.Context(0)
+    conv.estimate_filter_device(ctx, g, d, maxiter=2)        # FFT plans and kernels built once
+    times, res = [], None
+    for _ in range(max(args.steps, 1)):
+        t0 = time.perf_counter()
+        res = conv.estimate_filter_device(ctx, g, d, maxiter=100)
+        times.append(time.perf_counter() - t0)
+    ctx.close()
+    sd, gs, it, stop = res
+    dev_s = min(times)
+    out = {"device": {"seconds": round(dev_s, 4), "iterations": it, "istop": stop,
+                      "ms_per_iteration": round(dev_s / max(it, 1) * 1e3, 3),
+                      "relative_residual": float(np.linalg.norm(gs - d) / np.linalg.norm(d))}}
+    if not args.no_cpu:
+        host_iters = 3
+        pa2 = conv.Pconv(dsize=(nt, ntr), ssize=(nt,), gsize=(nt, ntr), g=g)
+        t0 = time.perf_counter()
+        ref = lsmr(conv.operator(pa2, "s"), d.ravel(order="F"), atol=conv.LSMR_TOL, btol=conv.LSMR_TOL,
+                   conlim=conv.LSMR_CONLIM, maxiter=host_iters)
+        host_s = time.perf_counter() - t0
+        out["host_scipy"] = {"iterations_timed": int(ref[2]), "seconds": round(host_s, 3),
+                             "ms_per_iteration": round(host_s / max(int(ref[2]), 1) * 1e3, 1),
+                             "threads": os.cpu_count()}
+    return {"metric": "source-signature estimation seconds per supersource", "value": round(dev_s, 4), "unit": "s",
+            "n_gpus": 1, "steps": len(times), "warmup": 1, "ms_per_step": round(dev_s * 1e3, 2),
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "LSMR (at most 100 iterations) for one filter shared by 2403 traces of 4000 "
+                                   "samples (the 2001^2 elastic workload's records), FFT length 8000",
+                       "samples": nt, "traces": ntr}, **out}
 
 
 def run_serve_jobs(args, G, _lib):
@@ -696,7 +749,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt", type=int, default=0, help="time steps (default: the workload's own)")
     ap.add_argument("--workload", choices=["elastic2d", "acoustic3d", "elastic3d", "gradient3d", "gradient3d_elastic",
-                                           "serve", "serve3d", "serve_jobs", "fwi_optimizer"], default="elastic2d")
+                                           "serve", "serve3d", "serve_jobs", "fwi_optimizer", "source_estimation"],
+                    default="elastic2d")
     ap.add_argument("--opt-n", type=int, default=64_000_000, help="fwi_optimizer: parameters")
     ap.add_argument("--grad-fraction", type=float, default=0.25, help="serve3d: share of gradient requests")
     ap.add_argument("--workers", type=int, default=2, help="serve3d, serve_jobs: jobs in flight")
@@ -754,6 +808,9 @@ def main():
         return
     if args.workload == "fwi_optimizer":
         print(json.dumps(run_fwi_optimizer(args, G, _lib)), flush=True)
+        return
+    if args.workload == "source_estimation":
+        print(json.dumps(run_source_estimation(args, G, _lib)), flush=True)
         return
     if args.workload == "serve_jobs":
         if args.serve_rate is None:

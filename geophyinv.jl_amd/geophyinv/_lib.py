@@ -163,6 +163,11 @@ SIGNATURES = {
     "gpi_lbfgs_reset": (C.c_int, [C.c_void_p]),
     "gpi_lbfgs_direction": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    "gpi_conv_apply": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_double), C.c_int32,
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "gpi_conv_lsmr": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                C.c_int32, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double),
+                                C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "gpi_last_timing": (C.c_int, [C.c_void_p, C.POINTER(gpi_timing)]),
     "gpi_set_pass_sampling": (C.c_int, [C.c_void_p, C.c_int32]),
 }

@@ -9,7 +9,9 @@ Defaults: d and g have only non-negative lags, s has as many negative as positiv
 ``len(s) + len(g) - 1`` (DSP.nextfastfft), so the circular convolution is the linear one.
 A dimension beyond the first is a trace axis; when one of the three signals has a single
 trace it is shared by all traces (and the products into it are summed over traces),
-Conv.jl:296-420.  Work is done on the host in FP64 with NumPy's real FFTs.
+Conv.jl:296-420.  ``Pconv`` works on the host in FP64 with NumPy's real FFTs;
+``apply_device`` and ``estimate_filter_device`` run the shared-filter operator and its
+LSMR solve on the GPU (``gpi_conv_apply``, ``gpi_conv_lsmr``: batched hipFFT, FP64).
 """
 from __future__ import annotations
 
@@ -148,16 +150,78 @@ def operator(pa: Pconv, unknown: str):
     return LinearOperator((m, n), matvec=fw, rmatvec=bk, dtype=np.float64)
 
 
-def estimate_filter(g: np.ndarray, d: np.ndarray, maxiter: int = 100, tol: float | None = None):
+# The stopping tolerances of IterativeSolvers.lsmr! (version 0.9.4, pinned by the
+# reference's Manifest.toml and not vendored; its documented keyword defaults, those of
+# Fong and Saunders' LSMR): atol = btol = 1e-6, conlim = 1e8.  update!(paw, pac) passes
+# only maxiter = 100 (src/fwi/fwi.jl:52).
+LSMR_TOL = 1e-6
+LSMR_CONLIM = 1e8
+
+
+def estimate_filter(g: np.ndarray, d: np.ndarray, maxiter: int = 100, tol: float | None = None,
+                    conlim: float = LSMR_CONLIM):
     """The filter s minimising ||g * s - d|| by LSMR from zero (IterativeSolvers.lsmr!,
-    maxiter 100; its default tolerances sqrt(eps) of the Float32 number type), and g * s.
-    ``g``, ``d``: nt x ntraces; s has nt samples with symmetric lags."""
+    maxiter 100, its default tolerances), and g * s, on the host
+    (SciPy's LSMR on ``operator``).  ``g``, ``d``: nt x ntraces; s has nt samples with
+    symmetric lags."""
     from scipy.sparse.linalg import lsmr
 
     nt = g.shape[0]
     pa = Pconv(dsize=d.shape, ssize=(nt,), gsize=g.shape, g=g)
     A = operator(pa, "s")
-    t = float(np.sqrt(np.finfo(np.float32).eps)) if tol is None else tol
-    w = lsmr(A, np.asarray(d, np.float64).ravel(order="F"), atol=t, btol=t, maxiter=maxiter)[0]
+    t = LSMR_TOL if tol is None else tol
+    w = lsmr(A, np.asarray(d, np.float64).ravel(order="F"), atol=t, btol=t, conlim=conlim, maxiter=maxiter)[0]
     pa.s = w
     return w, pa.mod_D()
+
+
+def _traces(a: np.ndarray) -> np.ndarray:
+    a = np.asfortranarray(a, dtype=np.float64)
+    return a.reshape(a.shape[0], -1, order="F")
+
+
+def apply_device(ctx, g: np.ndarray, x: np.ndarray, mode: str) -> np.ndarray:
+    """The shared-filter operator on the GPU: ``mode="D"`` d = g * x (x: nt filter samples,
+    symmetric lags), ``mode="S"`` the adjoint s = sum over traces of x correlated with g
+    (x: nt x ntraces); as ``Pconv.mod_D`` / ``mod_S`` with dsize = gsize = (nt, ntraces)."""
+    from . import _lib
+
+    g = _traces(g)
+    nt, ntr = g.shape
+    if mode == "D":
+        xin = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        if xin.size != nt:
+            raise ValueError("apply_device: the filter needs nt samples")
+        out = np.zeros((nt, ntr), np.float64, order="F")
+    elif mode == "S":
+        xin = _traces(x)
+        if xin.shape != (nt, ntr):
+            raise ValueError("apply_device: data and g differ in shape")
+        out = np.zeros(nt, np.float64)
+    else:
+        raise ValueError("mode must be 'D' or 'S'")
+    ctx.check(ctx.lib.gpi_conv_apply(ctx.h, nt, ntr, _lib.dptr(g), 0 if mode == "D" else 1, _lib.dptr(xin),
+                                     _lib.dptr(out)))
+    return out
+
+
+def estimate_filter_device(ctx, g: np.ndarray, d: np.ndarray, maxiter: int = 100, tol: float | None = None,
+                           conlim: float = LSMR_CONLIM):
+    """``estimate_filter`` on the GPU (``gpi_conv_lsmr``): the operator by batched hipFFT
+    and LSMR with the long vectors in HBM, in FP64.  Returns (s, g * s, iterations, istop)."""
+    import ctypes as C
+
+    from . import _lib
+
+    g = _traces(g)
+    dd = _traces(d)
+    if dd.shape != g.shape:
+        raise ValueError("estimate_filter_device: g and d differ in shape")
+    nt, ntr = g.shape
+    t = LSMR_TOL if tol is None else float(tol)
+    s = np.zeros(nt, np.float64)
+    gs = np.zeros((nt, ntr), np.float64, order="F")
+    it, stop = C.c_int32(0), C.c_int32(0)
+    ctx.check(ctx.lib.gpi_conv_lsmr(ctx.h, nt, ntr, _lib.dptr(g), _lib.dptr(dd), int(maxiter), t, t, float(conlim),
+                                    _lib.dptr(s), _lib.dptr(gs), C.byref(it), C.byref(stop)))
+    return s, gs, int(it.value), int(stop.value)

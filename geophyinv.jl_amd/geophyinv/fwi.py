@@ -252,9 +252,9 @@ class SeisInvExpt:
         """``update!(paw, pac)`` (src/fwi/fwi.jl:37-64): model the data at the current model,
         then per supersource estimate the filter w that best maps the original observed data
         onto the modelled data (LSMR on the convolution operator with g = dobs0, all fields'
-        traces sharing one w) and replace the observed data by dobs0 * w.  Returns the
-        misfits before and after."""
-        from .conv import estimate_filter
+        traces sharing one w; on the GPU, ``conv.estimate_filter_device``) and replace the
+        observed data by dobs0 * w.  Returns the misfits before and after."""
+        from .conv import estimate_filter_device
 
         m = self.get_modelvector()
         loss_initial = self.lossvalue(m)
@@ -262,7 +262,7 @@ class SeisInvExpt:
             fields = list(dcal.fields)
             g = np.concatenate([np.asarray(d0.d[f], np.float64) for f in fields], axis=1)
             dc = np.concatenate([np.asarray(dcal.d[f], np.float64) for f in fields], axis=1)
-            _, conv = estimate_filter(g, dc, maxiter=maxiter)
+            _, conv, _, _ = estimate_filter_device(self.ctx, g, dc, maxiter=maxiter)
             off = 0
             for f in fields:
                 n = d.d[f].shape[1]

@@ -456,6 +456,21 @@ void gpi_lbfgs_destroy(gpi_lbfgs* lbfgs);
 int gpi_lbfgs_reset(gpi_lbfgs* lbfgs);   /* forget the history and the previous point */
 int gpi_lbfgs_direction(gpi_lbfgs* lbfgs, const double* m, const double* g, double* d, int32_t* pairs);
 
+/* Source-signature estimation (update!(paw, pac), src/fwi/fwi.jl:37-64, with the
+ * convolution operator of src/Conv.jl:30-480).  g and d are nt x ntr (time fastest, FP64,
+ * host); the filter s has nt samples: nnl = nt - 1 - (nt - 1) / 2 negative lags (most
+ * negative first), lag 0 at index nnl, then (nt - 1) / 2 positive lags.  d = g * s
+ * convolves every trace of g with the shared s (lags 0 .. nt-1 kept).
+ * gpi_conv_apply: mode 0 out (nt x ntr) = g * in (nt); mode 1 out (nt) = the adjoint, the
+ * sum over traces of in (nt x ntr) correlated with g (Conv.mod! with D() and S()).
+ * gpi_conv_lsmr: s = argmin ||g * s - d|| by LSMR from zero (IterativeSolvers.lsmr! on
+ * Conv.operator(pa, G()), maxiter, atol, btol, conlim as there), optionally gs = g * s;
+ * *iters and *istop report the iterations and the stopping rule (1-7 as in LSMR). */
+int gpi_conv_apply(gpi_ctx* ctx, int64_t nt, int64_t ntr, const double* g, int32_t mode, const double* in,
+                   double* out);
+int gpi_conv_lsmr(gpi_ctx* ctx, int64_t nt, int64_t ntr, const double* g, const double* d, int32_t maxiter,
+                  double atol, double btol, double conlim, double* s, double* gs, int32_t* iters, int32_t* istop);
+
 /* Device timing of the last gpi_forward on this context. */
 int gpi_last_timing(const gpi_ctx* ctx, gpi_timing* out);
 

@@ -58,6 +58,11 @@ struct dim3 {
     constexpr dim3(uint32_t a = 1, uint32_t b = 1, uint32_t c = 1) : x(a), y(b), z(c) {}
 };
 
+// HIP's vector of two doubles (complex samples of the FFT work)
+struct alignas(16) double2 {
+    double x, y;
+};
+
 namespace emu {
 struct Stream;
 struct Event;

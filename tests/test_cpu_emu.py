@@ -124,6 +124,92 @@ def test_elastic_orders_equal_the_oracle(built):
     assert r.returncode == 0 and "elastic cases equal the oracle" in r.stdout, (r.returncode, r.stderr[-3000:])
 
 
+CONV = r"""
+import sys
+sys.path[:0] = [{pkg!r}, {tests!r}]
+from geophyinv import _lib
+_lib.load({lib!r})
+import numpy as np
+from scipy.sparse.linalg import lsmr
+from geophyinv import conv
+ctx = _lib.Context(0)
+rng = np.random.Generator(np.random.PCG64(71))
+for nt, ntr in ((1, 1), (2, 3), (57, 4), (300, 6)):
+    g = rng.standard_normal((nt, ntr))
+    s = rng.standard_normal(nt)
+    d = rng.standard_normal((nt, ntr))
+    pa = conv.Pconv(dsize=(nt, ntr), ssize=(nt,), gsize=(nt, ntr), g=g, s=s)
+    ref_d = pa.mod_D().copy()
+    pa.d = d
+    ref_s = pa.mod_S().copy()
+    dev_d = conv.apply_device(ctx, g, s, "D")
+    dev_s = conv.apply_device(ctx, g, d, "S")
+    assert np.linalg.norm(dev_d - ref_d) <= 1e-12 * np.linalg.norm(ref_d), (nt, ntr)
+    assert np.linalg.norm(dev_s - ref_s) <= 1e-12 * np.linalg.norm(ref_s), (nt, ntr)
+    assert abs(np.vdot(dev_d, d) - np.vdot(s, dev_s)) <= 1e-12 * abs(np.vdot(dev_d, d)), (nt, ntr)
+# LSMR against SciPy's on conv.operator, same tolerances.  White traces make a well
+# conditioned operator: same stopping rule, same iterations, same filter.  Band-limited
+# traces make an ill-conditioned one, on which LSMR's iterates depend on the operator's
+# rounding once orthogonality is lost: there the first iterations agree with SciPy's on
+# the host operator, 20 iterations agree with SciPy's LSMR driving the device operator,
+# and the full solve fits the data about as well as SciPy's.
+def lsmr_pair(g, d, maxiter):
+    nt, ntr = g.shape
+    sd, gs, it, stop = conv.estimate_filter_device(ctx, g, d, maxiter=maxiter)
+    pa2 = conv.Pconv(dsize=(nt, ntr), ssize=(nt,), gsize=(nt, ntr), g=g)
+    ref = lsmr(conv.operator(pa2, "s"), d.ravel(order="F"), atol=conv.LSMR_TOL, btol=conv.LSMR_TOL,
+               conlim=conv.LSMR_CONLIM, maxiter=maxiter)
+    pa2.s = sd
+    assert np.linalg.norm(gs - pa2.mod_D()) <= 1e-12 * np.linalg.norm(gs)
+    return sd, gs, it, stop, ref
+nt, ntr = 240, 5
+sl = (nt - 1) - (nt - 1) // 2
+w = np.zeros(nt); w[sl + 7] = 1.7; w[sl + 8] = -0.4
+g = rng.standard_normal((nt, ntr))
+d = conv.Pconv(dsize=(nt, ntr), ssize=(nt,), gsize=(nt, ntr), g=g, s=w).mod_D() + 1e-3 * rng.standard_normal((nt, ntr))
+for maxiter in (3, 100):
+    sd, gs, it, stop, ref = lsmr_pair(g, d, maxiter)
+    assert (stop, it) == (ref[1], ref[2]), (maxiter, stop, it, ref[1], ref[2])
+    assert np.linalg.norm(sd - ref[0]) <= 1e-9 * np.linalg.norm(ref[0]), (maxiter, np.linalg.norm(sd - ref[0]))
+assert stop in (1, 2) and np.linalg.norm(sd - w) < 0.01 * np.linalg.norm(w)
+t = np.arange(nt)
+g = np.stack([np.convolve(rng.standard_normal(nt), np.exp(-0.5 * ((t[:31] - 15) / 4.0) ** 2))[:nt]
+              for _ in range(ntr)], axis=1)
+d = conv.Pconv(dsize=(nt, ntr), ssize=(nt,), gsize=(nt, ntr), g=g, s=w).mod_D() + 1e-3 * rng.standard_normal((nt, ntr))
+sd, gs, it, stop, ref = lsmr_pair(g, d, 6)
+assert (stop, it) == (7, 6) == (ref[1], ref[2])
+assert np.linalg.norm(sd - ref[0]) <= 1e-12 * np.linalg.norm(ref[0])
+from scipy.sparse.linalg import LinearOperator
+Adev = LinearOperator((nt * ntr, nt), matvec=lambda x: conv.apply_device(ctx, g, x, "D").ravel(order="F"),
+                      rmatvec=lambda y: conv.apply_device(ctx, g, y.reshape(nt, ntr, order="F"), "S"),
+                      dtype=np.float64)
+sd, gs, it, stop = conv.estimate_filter_device(ctx, g, d, maxiter=20)
+ref = lsmr(Adev, d.ravel(order="F"), atol=conv.LSMR_TOL, btol=conv.LSMR_TOL, conlim=conv.LSMR_CONLIM, maxiter=20)
+assert (stop, it) == (ref[1], ref[2])
+assert np.linalg.norm(sd - ref[0]) <= 1e-9 * np.linalg.norm(ref[0]), np.linalg.norm(sd - ref[0])
+sd, gs, it, stop, ref = lsmr_pair(g, d, 100)
+assert np.linalg.norm(gs - d) <= 1.3 * ref[3], (np.linalg.norm(gs - d), ref[3])
+# nothing to fit: zero data gives the zero filter
+sd, gs, it, stop = conv.estimate_filter_device(ctx, g, np.zeros_like(d))
+assert not sd.any() and not gs.any() and it == 0
+print("convolution equals the host")
+"""
+
+
+def test_convolution_and_lsmr_equal_the_host(built):
+    """Source-signature estimation through the CPU build (its hipFFT stand-in): the
+    shared-filter convolution and its adjoint equal conv.Pconv (NumPy FFTs) to 1e-12 and
+    pass the dot-product test, for single-sample, short and longer signals; on a well
+    conditioned problem LSMR stops by the same rule after the same number of iterations as
+    SciPy's LSMR on conv.operator and returns the same filter to 1e-9 (recovering the
+    planted one); on an ill-conditioned one the first iterations agree with it, 20 agree
+    with SciPy's LSMR driving the device operator, and the full solve's residual is within
+    1.3x of SciPy's; zero data gives the zero filter."""
+    code = CONV.format(pkg=os.path.join(ROOT, "geophyinv.jl_amd"), tests=os.path.join(ROOT, "tests"), lib=built)
+    r = run([sys.executable, "-c", code])
+    assert r.returncode == 0 and "convolution equals the host" in r.stdout, (r.returncode, r.stderr[-3000:])
+
+
 JOBS = r"""
 import gc, sys
 sys.path[:0] = [{pkg!r}, {tests!r}]
