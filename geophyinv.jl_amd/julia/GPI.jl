@@ -447,4 +447,64 @@ function mod_x_proc!(pac, pap, activepw, src_flags)
     return nothing
 end
 
+# ---- the FWI outer loop (src/fwi/fwi.jl) -----------------------------------------------
+
+# The filter estimate of update!(paw, pac) (fwi.jl:42-57) in place of Conv.operator +
+# IterativeSolvers.lsmr!: g and d are nt x ntr (the supersource's fields side by side), s
+# has nt samples with Conv.Pconv's symmetric lags, gs = g * s.  Returns (iterations, istop).
+function conv_lsmr!(s::Vector{Float64}, gs::Matrix{Float64}, g::Matrix{Float64}, d::Matrix{Float64};
+                    maxiter::Integer = 100, atol::Real = 1e-6, btol::Real = 1e-6, conlim::Real = 1e8)
+    ctx = ctx!()
+    nt, ntr = size(g)
+    (size(d) == size(g) == size(gs) && length(s) == nt) || throw(DimensionMismatch("conv_lsmr!"))
+    iters, istop = Ref{Int32}(0), Ref{Int32}(0)
+    check(ctx, ccall((:gpi_conv_lsmr, lib), Cint,
+                     (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Int32, Float64, Float64, Float64,
+                      Ptr{Float64}, Ptr{Float64}, Ref{Int32}, Ref{Int32}),
+                     ctx, nt, ntr, g, d, maxiter, atol, btol, conlim, s, gs, iters, istop))
+    return iters[], istop[]
+end
+
+# apply_proj_matrix! (proj_mat.jl:208-226) on the device: y = x contracted with the dense
+# Float32 weight matrix of each axis (nin[a] x nout[a], as get_proj_matrix builds them).
+function interp_apply!(y::Array{Float64}, x::Array{Float64}, mats::Vector{Matrix{Float32}})
+    ctx = ctx!()
+    nd = length(mats)
+    nin = Int64[size(W, 1) for W in mats]
+    nout = Int64[size(W, 2) for W in mats]
+    (size(x) == Tuple(nin) && size(y) == Tuple(nout)) || throw(DimensionMismatch("interp_apply!"))
+    append!(nin, ones(Int64, 3 - nd))
+    append!(nout, ones(Int64, 3 - nd))
+    ptrs = Ptr{Float32}[pointer(W) for W in mats]
+    append!(ptrs, fill(Ptr{Float32}(C_NULL), 3 - nd))
+    GC.@preserve mats check(ctx, ccall((:gpi_interp_apply, lib), Cint,
+                                       (Ptr{Cvoid}, Int32, Ptr{Int64}, Ptr{Int64}, Ptr{Ptr{Float32}}, Ptr{Float64},
+                                        Ptr{Float64}), ctx, nd, nin, nout, ptrs, x, y))
+    return y
+end
+
+# L-BFGS directions with the pair history in HBM, for an optimizer loop in place of
+# Optim.LBFGS (fwi.jl:66-75): direction!(lb, d, m, g) returns the pairs used, d = -H g.
+mutable struct LBFGS
+    h::Ptr{Cvoid}
+end
+
+function LBFGS(n::Integer, memory::Integer = 10)
+    ctx = ctx!()
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ctx, ccall((:gpi_lbfgs_create, lib), Cint, (Ptr{Cvoid}, Int64, Int32, Ref{Ptr{Cvoid}}), ctx, n, memory, h))
+    lb = LBFGS(h[])
+    finalizer(x -> ccall((:gpi_lbfgs_destroy, lib), Cvoid, (Ptr{Cvoid},), x.h), lb)
+    return lb
+end
+
+function direction!(lb::LBFGS, d::Vector{Float64}, m::Vector{Float64}, g::Vector{Float64})
+    pairs = Ref{Int32}(0)
+    check(ctx!(), ccall((:gpi_lbfgs_direction, lib), Cint,
+                        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{Int32}), lb.h, m, g, d, pairs))
+    return pairs[]
+end
+
+reset!(lb::LBFGS) = check(ctx!(), ccall((:gpi_lbfgs_reset, lib), Cint, (Ptr{Cvoid},), lb.h))
+
 end # module GPI

@@ -93,3 +93,46 @@ def test_binding_passes_mod_and_dmod():
 def test_binding_has_no_placeholder_bodies():
     # splats (f(x...)) are code; a line that is only an ellipsis, or an ellipsis comment, is not
     assert not re.search(r"^\s*(#\s*)?\.\.\.", JL, re.M)
+
+
+def c_prototypes():
+    """Entry point name -> number of parameters, from the header's prototypes."""
+    out = {}
+    text = re.sub(r"/\*.*?\*/", "", HEADER, flags=re.S)
+    for m in re.finditer(r"\b(?:int|void|const char\s*\*|uint32_t)\s+(gpi_\w+)\s*\((.*?)\);", text, re.S):
+        params = m.group(2).strip()
+        out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def jl_ccalls():
+    """(name, number of argument types) of every ccall in the binding."""
+    calls = []
+    for m in re.finditer(r"ccall\(\(:(gpi_\w+), lib\),\s*\w+,\s*\(", JL):
+        depth, i = 1, m.end()
+        while depth:
+            depth += {"(": 1, ")": -1}.get(JL[i], 0)
+            i += 1
+        types = JL[m.end():i - 1]
+        level, n = 0, 1 if types.strip() else 0
+        for ch in types:
+            level += {"{": 1, "}": -1, "(": 1, ")": -1}.get(ch, 0)
+            if ch == "," and level == 0:
+                n += 1
+        if types.strip().endswith(","):
+            n -= 1
+        calls.append((m.group(1), n))
+    return calls
+
+
+def test_every_ccall_matches_a_header_prototype():
+    """Each entry point the binding calls is declared in the header with as many parameters
+    as the ccall passes types (the FWI outer loop's included: conv_lsmr!, interp_apply!,
+    the L-BFGS handle)."""
+    protos = c_prototypes()
+    calls = jl_ccalls()
+    names = {n for n, _ in calls}
+    assert {"gpi_forward", "gpi_adjoint", "gpi_conv_lsmr", "gpi_interp_apply", "gpi_lbfgs_direction"} <= names
+    for name, n in calls:
+        assert name in protos, name
+        assert protos[name] == n, (name, protos[name], n)
