@@ -473,14 +473,11 @@ __device__ __forceinline__ int shot_of_block(const int* __restrict__ first, int 
     return __builtin_amdgcn_readfirstlane(lo);
 }
 
+// Workgroup b of a batched pass at batch step `step` (the shot's own step it = step - start).
 template <int O, int PHYS, int TY, int RX>
-__global__ __launch_bounds__(64 * TY) void vel2d_batch(const BatchShot2D* __restrict__ D, const int* __restrict__ first,
-                                                       const int* __restrict__ nshots,
-                                                       const int64_t* __restrict__ step0, int koff)
+__device__ __forceinline__ void vel_batch_block(const BatchShot2D* __restrict__ D, const int* __restrict__ first, int n,
+                                                int64_t step, int b)
 {
-    const int64_t step = *step0 + koff;
-    const int n = *nshots;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= first[n]) return;
     const int s = shot_of_block(first, n, b);
     const BatchShot2D& S = D[s];
@@ -496,13 +493,9 @@ __global__ __launch_bounds__(64 * TY) void vel2d_batch(const BatchShot2D* __rest
 }
 
 template <int O, int PHYS, int TY, int RX>
-__global__ __launch_bounds__(64 * TY) void str2d_batch(const BatchShot2D* __restrict__ D, const int* __restrict__ first,
-                                                       const int* __restrict__ nshots,
-                                                       const int64_t* __restrict__ step0, int koff)
+__device__ __forceinline__ void str_batch_block(const BatchShot2D* __restrict__ D, const int* __restrict__ first, int n,
+                                                int64_t step, int b)
 {
-    const int64_t step = *step0 + koff;
-    const int n = *nshots;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= first[n]) return;
     const int s = shot_of_block(first, n, b);
     const BatchShot2D& S = D[s];
@@ -515,6 +508,22 @@ __global__ __launch_bounds__(64 * TY) void str2d_batch(const BatchShot2D* __rest
         return;
     }
     str_tile<O, PHYS, TY, RX>(S.sp, S.xs, it, S.xs.ninj, lb);
+}
+
+template <int O, int PHYS, int TY, int RX>
+__global__ __launch_bounds__(64 * TY) void vel2d_batch(const BatchShot2D* __restrict__ D, const int* __restrict__ first,
+                                                       const int* __restrict__ nshots,
+                                                       const int64_t* __restrict__ step0, int koff)
+{
+    vel_batch_block<O, PHYS, TY, RX>(D, first, *nshots, *step0 + koff, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+template <int O, int PHYS, int TY, int RX>
+__global__ __launch_bounds__(64 * TY) void str2d_batch(const BatchShot2D* __restrict__ D, const int* __restrict__ first,
+                                                       const int* __restrict__ nshots,
+                                                       const int64_t* __restrict__ step0, int koff)
+{
+    str_batch_block<O, PHYS, TY, RX>(D, first, *nshots, *step0 + koff, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 __global__ void batch_set_step_kernel(int64_t* step0, int64_t step)

@@ -104,6 +104,21 @@ def test_server_direct_launch_equals_plain(gpu_ctx):
     assert st["completed"] == 6 and st["graph_chunks"] == 0 and st["graph_captures"] == 0
 
 
+@pytest.mark.skip(reason="persistent cooperative chunk launches hung on the MI355X (the first case, 2-D "
+                         "acoustic order 2, made no progress for 180 s); no cause in this code was shown, so "
+                         "the mode was taken out of the library and this case stays off the GPU")
+@pytest.mark.parametrize("physics,order", [(0, 2), (0, 4), (1, 2)])
+def test_server_persistent_launch_equals_plain(gpu_ctx, physics, order):
+    """Every chunk as one cooperative launch (grid barriers between the passes): each served
+    shot equals the plain path, with staggered admission (at most 3 resident) and the
+    autotuner switching workgroup shapes between chunks."""
+    shots = [shot(gpu_ctx, 500 + k, physics=physics, order=order) for k in range(5)]
+    st, _ = run_server(gpu_ctx, shots, chunk_steps=16, max_shots=3, persistent=True)
+    assert st["completed"] == 5 and st["failed"] == 0
+    assert st["graph_chunks"] == 0 and st["graph_captures"] == 0
+    assert st["launches"] < 2 * st["steps"]   # one launch per chunk
+
+
 def test_server_autotune_orders_and_staggered_admission(gpu_ctx):
     shots = [shot(gpu_ctx, 200 + k, order=4) for k in range(8)]
     st, worst = run_server(gpu_ctx, shots, autotune=True, chunk_steps=8, max_shots=3)
