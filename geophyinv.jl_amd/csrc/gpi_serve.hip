@@ -10,11 +10,14 @@
 // only after both streams have drained, so no queued launch can reference freed memory.
 // At most two chunks are in flight, so admission and copy-back overlap the device work.
 // The workgroup shape of the batched kernels is chosen per batch size by timing each
-// candidate on live chunks.  A full chunk is one hipGraphLaunch of a captured graph of its
-// 2 x chunk_steps launches: the kernels read the batch step and shot count from device
-// memory and the launches cover a bucketed workgroup count (extra workgroups return), so
-// one graph per shape and bucket serves every batch composition (least recently used
-// graphs evicted).
+// candidate on live chunks; a candidate's time is read when its chunk is accounted anyway
+// (the older of the two chunks in flight), so testing never drains the stream.  A full
+// chunk whose layout (shape and workgroup buckets) has run before is one hipGraphLaunch of
+// a captured graph of its 2 x chunk_steps launches: the kernels read the batch step and
+// shot count from device memory and the launches cover a bucketed workgroup count (extra
+// workgroups return), so one graph per shape and bucket serves every batch composition
+// (least recently used graphs evicted); a layout's first chunk is launched pass by pass,
+// so layouts that never recur are not captured.
 //
 // No reference counterpart: the reference propagates supersources one after another on
 // each worker (src/fdtd/propagate.jl:100-106, :138-261).
@@ -28,6 +31,7 @@
 #include <memory>
 #include <mutex>
 #include <thread>
+#include <tuple>
 
 #include "fdtd2d.h"
 #include "gpi_util.h"
@@ -186,7 +190,9 @@ struct gpi_server {
         uint64_t used;
     };
     static constexpr size_t kMaxGraphs = 16;
+    static constexpr int kAdmitPerPass = 8;    // shots admitted between two chunk launches
     std::vector<ChunkGraph> graphs;
+    std::map<std::tuple<int, int, int>, int> layout_seen;   // chunks run per layout
     std::vector<hipGraphExec_t> graph_grave;   // evicted; destroyed once the streams drain
     uint64_t graph_clock = 0;
     hipStream_t cap = nullptr;                 // capture stream (never runs work)
@@ -288,17 +294,20 @@ int launch_bucket(int nb)
     return v;
 }
 
-// The executable graph of a full chunk for this shape and workgroup buckets, captured on
-// first use.  Everything its launches read (table, shot count, step) lives at addresses
-// fixed for the server's life and is written before the chunk that needs it.
+// The executable graph of a full chunk for this shape and workgroup buckets, captured when
+// the layout runs for the second time (*out stays null the first time: launch it pass by
+// pass).  Everything its launches read (table, shot count, step) lives at addresses fixed
+// for the server's life and is written before the chunk that needs it.
 int gpi_server::chunk_graph(int shape, int bv, int bs, hipGraphExec_t* out)
 {
+    *out = nullptr;
     for (ChunkGraph& g : graphs)
         if (g.shape == shape && g.nbv == bv && g.nbs == bs) {
             g.used = ++graph_clock;
             *out = g.exec;
             return GPI_OK;
         }
+    if (layout_seen[std::make_tuple(shape, bv, bs)]++ == 0) return GPI_OK;
     hipGraph_t gr = nullptr;
     HIPTRY(cc, hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed));
     launch_chunk(TILE_SHAPES[shape], bv, bs, o.chunk_steps, cap);
@@ -520,10 +529,12 @@ void gpi_server::run()
     bool dirty = false;
     int nbv = 0, nbs = 0;
     int shape = o.autotune ? 0 : o.tile;
-    // autotuning state: class of the current batch, candidate under test, measured times
+    int table_shape = -1;       // the shape the uploaded table's workgroup grids are for
+    // autotuning state: class of the current batch, next candidate to launch (-1: none),
+    // per-step times measured for the class, and the candidate each in-flight chunk tests
     int cls = -1, testing = -1;
-    double tbest = 0.0;
-    int ibest = 0;
+    double tms[NTILE_SHAPES];
+    int ptest_shape[2] = {-1, -1}, ptest_cls[2] = {-1, -1};
     int64_t pend[2] = {0, 0};   // steps of an in-flight chunk not yet accounted
     // device time of a finished chunk (also the autotuner's measurement)
     auto account = [&](hipEvent_t a, hipEvent_t b, int64_t kk) {
@@ -533,6 +544,22 @@ void gpi_server::run()
             busy_ms += ms;
         }
         return (double)ms / (double)std::max<int64_t>(kk, 1);
+    };
+    // a finished chunk of parity slot k: its device time, and the autotuner's measurement
+    auto account_slot = [&](int k) {
+        const double per = account(ck_ev[k][0], ck_ev[k][1], pend[k]);
+        pend[k] = 0;
+        const int ts = ptest_shape[k];
+        ptest_shape[k] = -1;
+        if (ts < 0 || ptest_cls[k] != cls || best_shape.count(cls)) return;
+        tms[ts] = per;
+        int best = 0;
+        for (int q = 0; q < NTILE_SHAPES; ++q) {
+            if (tms[q] < 0.0) return;   // a candidate still in flight or not launched yet
+            if (tms[q] < tms[best]) best = q;
+        }
+        best_shape[cls] = best;
+        if (testing < 0) shape = best;
     };
     auto shot_fail_all = [&](int code, const std::string& msg) {
         for (Slot* s : active) {
@@ -559,7 +586,9 @@ void gpi_server::run()
                 cv.wait(lk, [&] { return stop || !queue.empty(); });
                 if (stop && queue.empty()) break;
             }
-            while (!queue.empty() && (int)(active.size() + take.size()) < o.max_shots) {
+            // a few shots per pass, so chunks start while the rest of a backlog is admitted
+            while (!queue.empty() && (int)(active.size() + take.size()) < o.max_shots &&
+                   (int)take.size() < kAdmitPerPass) {
                 const uint64_t need = queue.front()->est;
                 if (used + grave_bytes + need > budget && !(active.empty() && take.empty() && used == 0)) {
                     blocked = true;
@@ -621,7 +650,7 @@ void gpi_server::run()
         if (dirty && o.autotune) {
             double cells = 0.0;
             for (Slot* s : active) cells += (double)s->m->g.n[0] * (double)s->m->g.n[2];
-            const int c = (int)std::floor(std::log2(std::max(cells, 1.0)) * 2.0);
+            const int c = (int)std::floor(std::log2(std::max(cells, 1.0)));   // classes a factor 2 apart
             if (c != cls) {
                 cls = c;
                 auto it = best_shape.find(c);
@@ -630,41 +659,49 @@ void gpi_server::run()
                     testing = -1;
                 } else {
                     testing = 0;
-                    shape = 0;
-                    tbest = 0.0;
-                    ibest = 0;
+                    std::fill(tms, tms + NTILE_SHAPES, -1.0);
                 }
             }
         }
-        const TileShape t = TILE_SHAPES[shape];
+        int this_test = -1;
+        if (testing >= 0) {   // the next candidate; its time is read when its chunk is accounted
+            shape = this_test = testing;
+            testing = testing + 1 < NTILE_SHAPES ? testing + 1 : -1;
+        }
+        // the chunk runs with this shape: accounting an older chunk below may pick the
+        // class's best shape, which the next chunk uses
+        const int cshape = shape;
+        const TileShape t = TILE_SHAPES[cshape];
         const double tl0 = now_ms();
-        if (dirty) {
+        if (dirty || cshape != table_shape) {   // the workgroup grids depend on the shape
             if (upload_table(t, &nbv, &nbs)) {
                 shot_fail_all(GPI_EHIP, cc->err);
                 continue;
             }
             dirty = false;
+            table_shape = cshape;
         }
         // ---- one chunk of steps ---------------------------------------------------------
         int64_t K = o.chunk_steps;
         for (Slot* s : active) K = std::min<int64_t>(K, s->start + s->r->nt + 1 - step);
         K = std::max<int64_t>(K, 1);
-        hipEvent_t* E = ck_ev[chunk_no & 1];
-        if (pend[chunk_no & 1] > 0) {   // at most two chunks in flight; account the older one
+        const int slot = (int)(chunk_no & 1);
+        hipEvent_t* E = ck_ev[slot];
+        if (pend[slot] > 0) {   // at most two chunks in flight; account the older one
             (void)hipEventSynchronize(E[1]);
-            account(E[0], E[1], pend[chunk_no & 1]);
-            pend[chunk_no & 1] = 0;
+            account_slot(slot);
         }
         (void)hipEventRecord(E[0], cc->stream);
         const int ns = (int)active.size();
         batch_set_step(dstep, step, cc->stream);
         bool graphed = false;
-        if (!o.direct_launch && K == o.chunk_steps) {
-            hipGraphExec_t ex = nullptr;
-            if (chunk_graph(shape, launch_bucket(nbv), launch_bucket(nbs), &ex)) {
-                shot_fail_all(GPI_EHIP, cc->err);
-                continue;
-            }
+        hipGraphExec_t ex = nullptr;
+        if (!o.direct_launch && K == o.chunk_steps &&
+            chunk_graph(cshape, launch_bucket(nbv), launch_bucket(nbs), &ex)) {
+            shot_fail_all(GPI_EHIP, cc->err);
+            continue;
+        }
+        if (ex) {
             const hipError_t ge = hipGraphLaunch(ex, cc->stream);
             if (ge != hipSuccess) {
                 shot_fail_all(GPI_EHIP, std::string("chunk graph launch: ") + hipGetErrorString(ge));
@@ -696,24 +733,9 @@ void gpi_server::run()
             last_ty = t.ty;
             last_rx = t.rx;
         }
-        if (testing >= 0) {   // the autotuner measures this chunk now
-            (void)hipEventSynchronize(E[1]);
-            const double per = account(E[0], E[1], K);
-            if (testing == 0 || per < tbest) {
-                tbest = per;
-                ibest = testing;
-            }
-            if (++testing >= NTILE_SHAPES) {
-                best_shape[cls] = ibest;
-                shape = ibest;
-                testing = -1;
-            } else {
-                shape = testing;
-            }
-            dirty = true;   // the workgroup grid depends on the shape
-        } else {
-            pend[(chunk_no - 1) & 1] = K;
-        }
+        pend[slot] = K;
+        ptest_shape[slot] = this_test;
+        ptest_cls[slot] = cls;
         // ---- shots past their trailing half step leave -----------------------------------
         for (size_t k = 0; k < active.size();) {
             Slot* s = active[k];
@@ -731,7 +753,7 @@ void gpi_server::run()
     }
     (void)hipStreamSynchronize(cc->stream);
     for (int k = 0; k < 2; ++k)
-        if (pend[k] > 0) account(ck_ev[k][0], ck_ev[k][1], pend[k]);
+        if (pend[k] > 0) account_slot(k);
     collect(true);
     publish_times();
 }
