@@ -380,8 +380,6 @@ __device__ __forceinline__ void vel3d_elastic_point(const VelParams3D& P, const 
     }
 }
 
-// The elastic passes march along x like the acoustic ones: workgroup (bx, by, chunk) runs
-// planes chunk*kc .. chunk*kc+kc-1 of its (z, y) tile one after another, so the planes
 // Elastic workgroups are 64 (z) x TY (y) x KZ (x) threads: the layer threadIdx.z of
 // workgroup chunk c updates plane c*KZ + threadIdx.z, so the planes whose x-stencils
 // share samples are updated by one workgroup at the same time and the shared samples are
@@ -539,8 +537,6 @@ __global__ __launch_bounds__(512) void str3d_elastic(StrParams3D P, PassExtras X
     str3d_elastic_point<O>(P, X, t.i, t.j, t.k, bt);
 }
 
-// x-planes per workgroup of the acoustic passes: GPI_3D_MARCH planes (default 4, measured
-// on 440^3 against 2, 8 and 16; 1 gives one plane per workgroup), at most the axis length
 // XCD-aware workgroup order of the 3-D passes: GPI_XCD_REMAP3 (default 0: measured on
 // 440^3 acoustic and 360^3 elastic, the plain order is 3-5 % faster in 3-D)
 int remap3()
@@ -553,6 +549,8 @@ int remap3()
     return r;
 }
 
+// x-planes per workgroup of the acoustic passes: GPI_3D_MARCH planes (default 4, measured
+// on 440^3 against 2, 8 and 16; 1 gives one plane per workgroup), at most the axis length
 int march_planes(int n)
 {
     static int kc = -1;

@@ -86,7 +86,7 @@ from geophyinv import _lib
 _lib.load({lib!r})
 import numpy as np
 import oracle as O
-from helpers import experiment, layered_elastic, rel_l2
+from helpers import experiment, experiment3d, layered_elastic, medium3d, rel_l2
 ctx = _lib.Context(0)
 FIN = ["tauxx", "tauzz", "tauxz", "vx", "vz"]
 cases = []
@@ -100,7 +100,16 @@ for order in (2, 4, 6, 8):
 # a column longer than 256 rows, sources near both ends
 cases.append(experiment(layered_elastic(nz=291, nx=18), nt=90, sfields=("vz", "tauxx"), rfields=("vx", "vz"), ns=2,
                         nr=5, order=2, npml=5, ctx=ctx, seed=61))
-for k, pa in enumerate(cases):
+# 3-D elastic: orders 2, 4, 8 with stress and velocity sources, and a free surface
+FIN3 = ["tauxx", "tauyy", "tauzz", "tauxy", "tauxz", "tauyz", "vx", "vy", "vz"]
+cases3 = [experiment3d(medium3d(1, n=(22, 18, 24) if order < 8 else (32, 30, 34)), nt=60, order=order,
+                       sfields=("tauxx", "vx"), rfields=("vx", "vy", "vz"), ctx=ctx, seed=30 + order)
+          for order in (2, 4, 8)]
+cases3.append(experiment3d(medium3d(1), nt=60, order=2, sfields=("tauzz", "vz"), rfields=("vx", "vz"),
+                           pml_faces=["zmax", "ymin", "ymax", "xmin", "xmax"], stressfree_faces=["zmin"], ctx=ctx,
+                           seed=41))
+cases = [(pa, FIN) for pa in cases] + [(pa, FIN3) for pa in cases3]
+for k, (pa, FIN) in enumerate(cases):
     orc = O.from_experiment(pa)
     orc.run()
     recs, fin = pa.run_shot(0, final_fields=FIN)
@@ -114,10 +123,11 @@ print("elastic cases equal the oracle", len(cases))
 
 
 def test_elastic_orders_equal_the_oracle(built):
-    """The 2-D elastic kernels against the oracle through the CPU build: orders 2-8, free
+    """The elastic kernels against the oracle through the CPU build: 2-D at orders 2-8, free
     surface with velocity and stress sources beside it, rigid faces, padded columns whose
-    length is not a multiple of 4, a column longer than one 256-row workgroup.  Records and
-    every final wavefield (ghost cells included) equal bit for bit."""
+    length is not a multiple of 4, a column longer than one 256-row workgroup; 3-D at orders
+    2, 4, 8 with stress and velocity sources and with a free surface.  Records and every
+    final wavefield (ghost cells included) equal bit for bit."""
     code = ELASTIC.format(pkg=os.path.join(ROOT, "geophyinv.jl_amd"), orc=os.path.join(ROOT, "oracle"),
                        tests=os.path.join(ROOT, "tests"), lib=built)
     r = run([sys.executable, "-c", code], timeout=900)
