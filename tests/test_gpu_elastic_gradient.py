@@ -114,7 +114,11 @@ def test_elastic_gradient_matches_finite_differences(gpu_ctx, case):
 
 def test_vp_vs_rho_chain_rule(gpu_ctx):
     """g w.r.t. (vp, vs, rho) from the (invlambda, invmu, rho) gradient against finite
-    differences of the misfit in the (vp, vs, rho) parameterization."""
+    differences of the misfit in the (vp, vs, rho) parameterization, to the 1e-3 bar.  The
+    step of each direction moves the misfit by about 1e-3 of itself (within 1e-3 .. 5e-2 of
+    the parameter's mean at the bump's peak): the rho direction at fixed velocities is about
+    150 times less sensitive than vp, and a fixed step leaves its misfit change near the
+    FP32 rounding noise of the wavefields."""
     true = layered_elastic(nz=40, nx=48, seed=12)
     start = true.copy()
     b = bump(true.vp.m.shape, [19.5, 23.5], 3.0).astype(np.float32)
@@ -124,7 +128,7 @@ def test_vp_vs_rho_chain_rule(gpu_ctx):
     loss = G.L2DistLoss()
     m = pa.model_vector()
     g = np.zeros_like(m)
-    G.gradient(g, m, loss, dobs, pa)
+    J0 = G.gradient(g, m, loss, dobs, pa)
     n = m.size // 3
     sl = pa.view_inner(pa.mod["rho"])
     gm = {k: g[i * n:(i + 1) * n].astype(np.float64) / pa.mod[k][sl].ravel(order="F").astype(np.float64)
@@ -145,10 +149,10 @@ def test_vp_vs_rho_chain_rule(gpu_ctx):
             p = {q: base[q].copy() for q in base}
             p[name] = p[name] + t * scale * d
             return J_of(*(p[q].astype(np.float32) for q in ("vp", "vs", "rho")))
-        e = 2e-3
-        fd = (8.0 * (J(e) - J(-e)) - (J(2 * e) - J(-2 * e))) / (12.0 * e)
         ad = float(np.dot(gv[k], (scale * d).ravel(order="F")))
+        e = float(np.clip(1e-3 * J0 / max(abs(ad), 1e-30), 1e-3, 5e-2))
+        fd = (8.0 * (J(e) - J(-e)) - (J(2 * e) - J(-2 * e))) / (12.0 * e)
         rel = abs(ad / fd - 1)
-        print(f"  {name}: fd {fd:.6e} adjoint {ad:.6e} rel {rel:.2e}", flush=True)
-        assert rel < 5e-3, (name, ad, fd)
+        print(f"  {name}: step {e:.1e} fd {fd:.6e} adjoint {ad:.6e} rel {rel:.2e}", flush=True)
+        assert rel < TOL, (name, ad, fd)
     pa.free()
