@@ -199,9 +199,21 @@ assert (stop, it) == (ref[1], ref[2])
 assert np.linalg.norm(sd - ref[0]) <= 1e-9 * np.linalg.norm(ref[0]), np.linalg.norm(sd - ref[0])
 sd, gs, it, stop, ref = lsmr_pair(g, d, 100)
 assert np.linalg.norm(gs - d) <= 1.3 * ref[3], (np.linalg.norm(gs - d), ref[3])
-# nothing to fit: zero data gives the zero filter
+# nothing to fit: zero data gives the zero filter; no iterations allowed: the zero filter
 sd, gs, it, stop = conv.estimate_filter_device(ctx, g, np.zeros_like(d))
 assert not sd.any() and not gs.any() and it == 0
+sd, gs, it, stop = conv.estimate_filter_device(ctx, g, d, maxiter=0)
+assert not sd.any() and it == 0 and stop == 0
+# bad arguments are refused with GPI_EINVAL and a message, nothing computed
+import ctypes as C
+out = np.zeros(nt)
+gf = np.asfortranarray(g)
+for args in [(0, ntr, 0), (nt, 0, 0), (nt, ntr, 2)]:
+    rc = ctx.lib.gpi_conv_apply(ctx.h, args[0], args[1], _lib.dptr(gf), args[2], _lib.dptr(out), _lib.dptr(out))
+    assert rc == -1 and b"gpi_conv_apply" in ctx.lib.gpi_last_error(ctx.h), (args, rc)
+rc = ctx.lib.gpi_conv_lsmr(ctx.h, nt, ntr, _lib.dptr(gf), _lib.dptr(gf), 10, 1e-6, 1e-6, 0.0, _lib.dptr(out), None,
+                           None, None)
+assert rc == -1 and b"gpi_conv_lsmr" in ctx.lib.gpi_last_error(ctx.h)
 print("convolution equals the host")
 """
 
@@ -214,7 +226,8 @@ def test_convolution_and_lsmr_equal_the_host(built):
     SciPy's LSMR on conv.operator and returns the same filter to 1e-9 (recovering the
     planted one); on an ill-conditioned one the first iterations agree with it, 20 agree
     with SciPy's LSMR driving the device operator, and the full solve's residual is within
-    1.3x of SciPy's; zero data gives the zero filter."""
+    1.3x of SciPy's; zero data and maxiter 0 give the zero filter; bad sizes, modes and
+    conlim are refused with GPI_EINVAL."""
     code = CONV.format(pkg=os.path.join(ROOT, "geophyinv.jl_amd"), tests=os.path.join(ROOT, "tests"), lib=built)
     r = run([sys.executable, "-c", code])
     assert r.returncode == 0 and "convolution equals the host" in r.stdout, (r.returncode, r.stderr[-3000:])
