@@ -24,7 +24,7 @@ struct VelParams3D {
     int64_t s1, s2;    // y and x strides of the pitched layout
     int gx, gy;        // workgroups along z and y; the grid's third factor runs over x
     int kc;            // x planes per workgroup (acoustic passes march along x)
-    int remap;         // XCD-aware workgroup order
+    int remap;         // workgroup order: 0 launch order, 1 XCD-aware; staged elastic passes: window run
     int xfast;         // elastic: consecutive workgroups walk along x (then z, y tiles)
     uint32_t pml_mask, rigid_mask;
     float dxI, dyI, dzI, dt;
@@ -47,7 +47,7 @@ struct StrParams3D {
     int64_t s1, s2;
     int gx, gy;
     int kc;            // x planes per workgroup (acoustic passes march along x)
-    int remap;         // XCD-aware workgroup order
+    int remap;         // workgroup order: 0 launch order, 1 XCD-aware; staged elastic passes: window run
     int xfast;         // elastic: consecutive workgroups walk along x (then z, y tiles)
     uint32_t pml_mask, free_mask;
     float dxI, dyI, dzI;

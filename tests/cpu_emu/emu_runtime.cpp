@@ -5,7 +5,7 @@
 // hipEventQuery/Synchronize and hipStreamWaitEvent wait for the record queued last before
 // the call, as HIP's do.  Kernels: the stream's worker splits the grid into workgroups
 // that it and a shared pool of host threads run; within a workgroup the threads run one
-// after another.  Memory: every allocation is its own mapping, placed against a guard page
+// after another, each on a fiber of its own so that a barrier can suspend it.  Memory: every allocation is its own mapping, placed against a guard page
 // after it (and one before); hipFree/hipHostFree wait for all streams (HIP's frees
 // synchronise the device) and hipFreeAsync waits for its stream position, then the pages
 // are made inaccessible and never mapped again, so a late access faults.  A fault prints
@@ -395,14 +395,138 @@ void launch(hipStream_t s, dim3 grid, dim3 block, const char* name, void (*run)(
     });
 }
 
+// ---- workgroup fibers ---------------------------------------------------------------------
+// Every thread of a workgroup runs on a fiber (a stack of its own on the worker's OS
+// thread).  The worker resumes the unfinished fibers in thread order; each runs until it
+// returns or reaches a barrier.  When a sweep over the fibers ends with fibers at the
+// barrier, every thread that has not returned is there, so the next sweep releases them.
+// Workgroup memory (__shared__, a thread_local of the OS thread) is therefore shared by
+// the fibers.
+
+extern "C" void emu_fiber_switch(void** save_sp, void* next_sp);
+// Saves the callee-saved registers of the System V x86-64 ABI on the current stack, stores
+// the stack pointer in *save_sp and resumes the fiber whose saved stack pointer is next_sp.
+asm(R"(
+    .text
+    .p2align 4
+    .globl emu_fiber_switch
+    .type emu_fiber_switch, @function
+emu_fiber_switch:
+    pushq %rbp
+    pushq %rbx
+    pushq %r12
+    pushq %r13
+    pushq %r14
+    pushq %r15
+    movq %rsp, (%rdi)
+    movq %rsi, %rsp
+    popq %r15
+    popq %r14
+    popq %r13
+    popq %r12
+    popq %rbx
+    popq %rbp
+    ret
+    .size emu_fiber_switch, .-emu_fiber_switch
+)");
+
+struct Fiber {
+    void* sp = nullptr;
+    dim3 tid;
+    bool done = false;
+};
+
+struct FiberSet {
+    std::vector<Fiber> f;
+    std::vector<char*> stacks;   // FIBER_STACK bytes each, above a guard page
+    std::vector<char> alt;       // this OS thread's signal stack (a fault on a fiber stack)
+    void* sched = nullptr;       // the worker's own stack while a fiber runs
+    int cur = -1;                // running fiber, -1 outside a workgroup
+    void (*one)(const void*) = nullptr;
+    const void* ctx = nullptr;
+};
+
+thread_local FiberSet tl_fibers;
+constexpr size_t FIBER_STACK = 256 * 1024;
+
+[[noreturn]] void fiber_main()
+{
+    FiberSet& w = tl_fibers;
+    w.one(w.ctx);
+    Fiber& me = w.f[(size_t)w.cur];
+    me.done = true;
+    emu_fiber_switch(&me.sp, w.sched);
+    __builtin_trap();   // a finished fiber is never resumed
+}
+
+void run_workgroup(void (*one)(const void*), const void* ctx)
+{
+    FiberSet& w = tl_fibers;
+    if (w.alt.empty()) {
+        w.alt.resize(1 << 16);
+        stack_t ss{};
+        ss.ss_sp = w.alt.data();
+        ss.ss_size = w.alt.size();
+        sigaltstack(&ss, nullptr);
+    }
+    const size_t n = (size_t)blockDim.x * blockDim.y * blockDim.z;
+    while (w.stacks.size() < n) {
+        char* m = (char*)mmap(nullptr, FIBER_STACK + PAGE, PROT_READ | PROT_WRITE,
+                              MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
+        if (m == MAP_FAILED || mprotect(m, PAGE, PROT_NONE) != 0) {
+            fprintf(stderr, "EMU: no memory for the fibers of %s\n", tl_kernel);
+            _exit(88);
+        }
+        w.stacks.push_back(m + PAGE);
+    }
+    w.f.assign(n, Fiber{});
+    size_t t = 0;
+    for (uint32_t tz = 0; tz < blockDim.z; ++tz)
+        for (uint32_t ty = 0; ty < blockDim.y; ++ty)
+            for (uint32_t tx = 0; tx < blockDim.x; ++tx, ++t) {
+                void** sp = (void**)(w.stacks[t] + FIBER_STACK);
+                *--sp = nullptr;                 // return address of fiber_main (never used)
+                *--sp = (void*)&fiber_main;      // where the first switch returns to
+                for (int r = 0; r < 6; ++r) *--sp = nullptr;   // rbp rbx r12 r13 r14 r15
+                w.f[t].sp = sp;
+                w.f[t].tid = dim3(tx, ty, tz);
+            }
+    w.one = one;
+    w.ctx = ctx;
+    // EMU_REVERSE=1 resumes the fibers last thread first, so a write that a barrier does not
+    // order before another thread's read of the same workgroup memory shows up as well
+    static const bool reverse = getenv("EMU_REVERSE") && atoi(getenv("EMU_REVERSE")) != 0;
+    for (;;) {
+        bool waiting = false;
+        for (size_t u = 0; u < n; ++u) {
+            t = reverse ? n - 1 - u : u;
+            Fiber& f = w.f[t];
+            if (f.done) continue;
+            w.cur = (int)t;
+            threadIdx = f.tid;
+            emu_fiber_switch(&w.sched, f.sp);
+            waiting |= !f.done;
+        }
+        if (!waiting) break;
+    }
+    w.cur = -1;
+}
+
 }  // namespace emu
 
 using namespace emu;
 
 void __syncthreads()
 {
-    fprintf(stderr, "EMU: __syncthreads in %s: workgroups run thread by thread here\n", tl_kernel);
-    _exit(87);
+    FiberSet& w = tl_fibers;
+    if (w.cur < 0) {
+        fprintf(stderr, "EMU: __syncthreads outside a workgroup (%s)\n", tl_kernel ? tl_kernel : "host code");
+        _exit(87);
+    }
+    Fiber& me = w.f[(size_t)w.cur];
+    const dim3 tid = me.tid;
+    emu_fiber_switch(&me.sp, w.sched);
+    threadIdx = tid;
 }
 
 // ---- runtime API -------------------------------------------------------------------------

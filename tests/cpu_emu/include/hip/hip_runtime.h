@@ -81,6 +81,29 @@ extern thread_local dim3 blockDim __attribute__((tls_model("initial-exec")));
 extern thread_local dim3 gridDim __attribute__((tls_model("initial-exec")));
 
 #define __builtin_amdgcn_readfirstlane(x) (x)
+
+// Raw buffer resources: base address and record count in bytes; a load at an offset at or
+// past the count reads 0 without touching memory, as the hardware's range check does
+// (soffset is added to the address but, as on gfx9, not to the checked offset).
+struct __amdgpu_buffer_rsrc_t {
+    const char* base;
+    uint32_t bytes;
+};
+namespace emu {
+inline __amdgpu_buffer_rsrc_t make_buffer_rsrc(const void* p, short, int bytes, int)
+{
+    return __amdgpu_buffer_rsrc_t{(const char*)p, (uint32_t)bytes};
+}
+inline uint32_t raw_buffer_load_b32(__amdgpu_buffer_rsrc_t r, int off, int soff, int)
+{
+    if ((uint32_t)off >= r.bytes) return 0u;
+    uint32_t v;
+    std::memcpy(&v, r.base + (uint32_t)off + (uint32_t)soff, 4);
+    return v;
+}
+}  // namespace emu
+#define __builtin_amdgcn_make_buffer_rsrc(p, stride, bytes, flags) ::emu::make_buffer_rsrc((p), (stride), (bytes), (flags))
+#define __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, aux) ::emu::raw_buffer_load_b32((r), (off), (soff), (aux))
 #define __builtin_nontemporal_store(v, p) (*(p) = (v))
 
 inline unsigned int __float_as_uint(float f)
@@ -98,7 +121,10 @@ inline float __uint_as_float(unsigned int u)
 using std::max;
 using std::min;
 
-// Workgroups run their threads one after another, so a barrier cannot be honoured.
+// Each thread of a workgroup runs as a fiber of the worker that runs the workgroup; a
+// barrier suspends the calling thread until every thread of its workgroup that has not
+// returned reaches it (threads that returned no longer take part, as ended waves do).
+// Between barriers the threads run in thread order, or last first with EMU_REVERSE=1.
 void __syncthreads();
 
 // ---- runtime API --------------------------------------------------------------------
@@ -158,6 +184,9 @@ namespace emu {
 // blockIdx/gridDim/blockDim are set (threadIdx is set per thread by `run`).
 void launch(hipStream_t s, dim3 grid, dim3 block, const char* name, void (*run)(void*), void* ctx,
             void (*drop)(void*));
+// Run `one` (one thread's body for the current threadIdx) for every thread of the current
+// workgroup, each on a fiber of its own so that __syncthreads can suspend it.
+void run_workgroup(void (*one)(const void*), const void* ctx);
 
 template <typename F>
 struct KernelSig;
@@ -170,16 +199,8 @@ struct KernelSig<void (*)(P...)> {
 template <auto K>
 struct KernelCall {
     typename KernelSig<decltype(K)>::Args args;
-    static void run(void* self)
-    {
-        const KernelCall* c = (const KernelCall*)self;
-        for (uint32_t tz = 0; tz < blockDim.z; ++tz)
-            for (uint32_t ty = 0; ty < blockDim.y; ++ty)
-                for (uint32_t tx = 0; tx < blockDim.x; ++tx) {
-                    threadIdx = dim3(tx, ty, tz);
-                    std::apply(K, c->args);
-                }
-    }
+    static void one(const void* self) { std::apply(K, ((const KernelCall*)self)->args); }
+    static void run(void* self) { run_workgroup(&one, self); }
     static void drop(void* self) { delete (KernelCall*)self; }
 };
 
