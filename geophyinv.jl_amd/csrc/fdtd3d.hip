@@ -83,6 +83,31 @@ __device__ __forceinline__ void load_s(float (&v)[O], const float* a, int off, i
     for (int m = 0; m < O; ++m) v[m] = a[off + m * stride];
 }
 
+// Loads through a buffer resource whose base is a wave-uniform plane pointer: the lane
+// offsets (bytes) are shared by all arrays of one tile geometry, so a loop over planes
+// keeps one offset register per geometry instead of a 64-bit address per array.  An
+// offset at or past `bytes` (the rest of the array from the base) reads 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float* a, int64_t s2, int p, int nplanes)
+{
+    const int64_t rest = 4 * s2 * (int64_t)(nplanes - p);
+    const int bytes = (int)(rest < 0x7fffffff ? rest : 0x7fffffff);
+    return __builtin_amdgcn_make_buffer_rsrc((void*)(a + s2 * p), (short)0, bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+
+// v, opaque to the optimiser: the addresses an update forms from its point (CPML strips,
+// rigid faces, injection) are then formed per plane of a marching loop instead of being
+// hoisted out of it, where each would hold a 64-bit register pair for the whole loop.
+__device__ __forceinline__ int opaque(int v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 struct Idx3 {
     int i, j, k;
 };
@@ -143,11 +168,18 @@ __global__ __launch_bounds__(512) void vel3d_acoustic(VelParams3D P, PassExtras 
     const bool row = i >= 0;
     const bool colx = row && i < P.nz - 2 * H && j < P.ny - 2 * H;   // this thread updates vx
     const int cx0 = (i + H) + s1 * (j + H);                     // its p column
+    const int np = P.nx + H;                                    // planes of every array
+    // lane offsets (bytes from a plane's start) for buffer loads with wave-uniform plane
+    // bases: the p column and the velocities at (i+h, j+h), the y-stencil of p from (i+h,
+    // j), its z-stencil from (i, j+h), the density coefficients at (i, j) (negative only
+    // for rows i < 0, which update nothing: such an offset reads 0)
+    const uint32_t lo = 4u * (uint32_t)cx0;
+    const uint32_t ly = lo - 4u * (uint32_t)(s1 * H);
+    const uint32_t lz = lo - 4u * (uint32_t)H;
+    const uint32_t lc = lz - 4u * (uint32_t)(s1 * H);
     float w[O];
 #pragma unroll
-    for (int m = 0; m < O - 1; ++m) w[m + 1] = colx ? p[cx0 + s2 * (k0 + m)] : 0.0f;
-    // operands of one plane; the next plane's are loaded before this plane's stores so
-    // that two planes of loads are in flight per wave
+    for (int m = 0; m < O - 1; ++m) w[m + 1] = bload(plane_rsrc(p, s2, k0 + m, np), lo);
     const bool ycol = row && i < P.nz - 2 * H && j < P.ny - H;   // vy column (plane k < nx-2h)
     const bool zcol = row && i < P.nz - H && j < P.ny - 2 * H;   // vz column (plane k < nx-2h)
     // (the precomputed density coefficients are read here: forming them from rho in this
@@ -156,68 +188,82 @@ __global__ __launch_bounds__(512) void vel3d_acoustic(VelParams3D P, PassExtras 
         float wn, vxo, cxo, vyo, cyo, vzo, czo;
         float py[O], pz[O];
     };
+    // Operands of plane k, loaded by every thread whether or not its updates run: a load
+    // whose condition could skip it would make every wait count the loads after it as
+    // possibly skipped, and such reads stay inside the arrays or read 0.
     auto fetch = [&](int k, Ops& L) {
-        const int o = (i + H) + s1 * (j + H) + s2 * (k + H);
-        const int c = i + s1 * j + s2 * k;
-        if (colx) {
-            L.wn = p[cx0 + s2 * (k + O - 1)];
-            L.vxo = P.vx[o];
-            L.cxo = P.cx[c];
-        }
-        if (ycol && k < P.nx - 2 * H) {
-            load_s<O>(L.py, p, (i + H) + s1 * j + s2 * (k + H), s1);
-            L.vyo = P.vy[o];
-            L.cyo = P.cy[c];
-        }
-        if (zcol && k < P.nx - 2 * H) {
-            load_s<O>(L.pz, p, i + s1 * (j + H) + s2 * (k + H), 1);
-            L.vzo = P.vz[o];
-            L.czo = P.cz[c];
-        }
+        const __amdgpu_buffer_rsrc_t pk = plane_rsrc(p, s2, k + H, np);
+        L.wn = bload(pk, lo);
+        L.vxo = bload(plane_rsrc(P.vx, s2, k + H, np), lo);
+        L.cxo = bload(plane_rsrc(P.cx, s2, k, np), lc);
+#pragma unroll
+        for (int m = 0; m < O; ++m) L.py[m] = bload(pk, ly + 4u * (uint32_t)(m * s1));
+        L.vyo = bload(plane_rsrc(P.vy, s2, k + H, np), lo);
+        L.cyo = bload(plane_rsrc(P.cy, s2, k, np), lc);
+#pragma unroll
+        for (int m = 0; m < O; ++m) L.pz[m] = bload(pk, lz + 4u * (uint32_t)m);
+        L.vzo = bload(plane_rsrc(P.vz, s2, k + H, np), lo);
+        L.czo = bload(plane_rsrc(P.cz, s2, k, np), lc);
     };
-    Ops cur{}, nxt{};
-    if (k0 < k1) fetch(k0, cur);
-    for (int k = k0; k < k1; ++k) {
-        if (k + 1 < k1) fetch(k + 1, nxt);
+    auto update = [&](int k, const Ops& cur) {
         const int bt = bx + P.gx * (by + P.gy * k);
-        const int o = (i + H) + s1 * (j + H) + s2 * (k + H);   // @inn of every velocity
+        const int iu = opaque(i), ju = opaque(j);
+        const int o = (iu + H) + s1 * (ju + H) + s2 * (k + H);   // @inn of every velocity
         // vx on (nz-2h, ny-2h, nx-h): dpdx = D_xi(p)
         if (colx) {
 #pragma unroll
             for (int m = 0; m < O - 1; ++m) w[m] = w[m + 1];
             w[O - 1] = cur.wn;
             float d = stencil<O>(w) * P.dxI;
-            d = cpml3_x(d, P.qx[0], i, j, k, P.npml, P.pml_mask);
+            d = cpml3_x(d, P.qx[0], iu, ju, k, P.npml, P.pml_mask);
             float nv = cur.vxo + cur.cxo * d;
-            if ((P.rigid_mask & GPI_FACE(2, 0)) && k == 0) P.vx[(i + H) + s1 * (j + H)] = -nv;
+            if ((P.rigid_mask & GPI_FACE(2, 0)) && k == 0) P.vx[(iu + H) + s1 * (ju + H)] = -nv;
             if ((P.rigid_mask & GPI_FACE(2, 1)) && k == P.nx - H - 1)
-                P.vx[(i + H) + s1 * (j + H) + s2 * (P.nx + H - 1)] = -nv;
+                P.vx[(iu + H) + s1 * (ju + H) + s2 * (P.nx + H - 1)] = -nv;
             if (X.ninj) nv = inject_velocity(X, INJ_VX, bt, o, nv, P.rho, P.dt);
             store_out<NT>(&P.vx[o], nv);
         }
         // vy on (nz-2h, ny-h, nx-2h): dpdy = D_yi(p)
         if (ycol && k < P.nx - 2 * H) {
             float d = stencil<O>(cur.py) * P.dyI;
-            d = cpml3_y(d, P.qy[0], i, j, k, P.npml, P.pml_mask);
+            d = cpml3_y(d, P.qy[0], iu, ju, k, P.npml, P.pml_mask);
             float nv = cur.vyo + cur.cyo * d;
-            if ((P.rigid_mask & GPI_FACE(1, 0)) && j == 0) P.vy[(i + H) + s2 * (k + H)] = -nv;
-            if ((P.rigid_mask & GPI_FACE(1, 1)) && j == P.ny - H - 1)
-                P.vy[(i + H) + s1 * (P.ny + H - 1) + s2 * (k + H)] = -nv;
+            if ((P.rigid_mask & GPI_FACE(1, 0)) && ju == 0) P.vy[(iu + H) + s2 * (k + H)] = -nv;
+            if ((P.rigid_mask & GPI_FACE(1, 1)) && ju == P.ny - H - 1)
+                P.vy[(iu + H) + s1 * (P.ny + H - 1) + s2 * (k + H)] = -nv;
             if (X.ninj) nv = inject_velocity(X, INJ_VY, bt, o, nv, P.rho, P.dt);
             store_out<NT>(&P.vy[o], nv);
         }
         // vz on (nz-h, ny-2h, nx-2h): dpdz = D_zi(p)
         if (zcol && k < P.nx - 2 * H) {
             float d = stencil<O>(cur.pz) * P.dzI;
-            d = cpml3_z(d, P.qz[0], i, j, k, P.npml, P.pml_mask);
+            d = cpml3_z(d, P.qz[0], iu, ju, k, P.npml, P.pml_mask);
             float nv = cur.vzo + cur.czo * d;
-            if ((P.rigid_mask & GPI_FACE(0, 0)) && i == 0) P.vz[s1 * (j + H) + s2 * (k + H)] = -nv;
-            if ((P.rigid_mask & GPI_FACE(0, 1)) && i == P.nz - H - 1)
-                P.vz[(P.nz + H - 1) + s1 * (j + H) + s2 * (k + H)] = -nv;
+            if ((P.rigid_mask & GPI_FACE(0, 0)) && iu == 0) P.vz[s1 * (ju + H) + s2 * (k + H)] = -nv;
+            if ((P.rigid_mask & GPI_FACE(0, 1)) && iu == P.nz - H - 1)
+                P.vz[(P.nz + H - 1) + s1 * (ju + H) + s2 * (k + H)] = -nv;
             if (X.ninj) nv = inject_velocity(X, INJ_VZ, bt, o, nv, P.rho, P.dt);
             store_out<NT>(&P.vz[o], nv);
         }
-        cur = nxt;
+    };
+    // two planes per trip with alternating operand sets: no register whose load may be
+    // pending is copied, and on every path the next plane's loads follow the current one's
+    // (vector memory loads complete in order, so an update waits only for its own plane)
+    Ops A{}, B{};
+    int k = k0;
+    if (k < k1) fetch(k, A);
+    for (; k + 2 < k1; k += 2) {
+        fetch(k + 1, B);
+        update(k, A);
+        fetch(k + 2, A);
+        update(k + 1, B);
+    }
+    if (k + 1 < k1) {
+        fetch(k + 1, B);
+        update(k, A);
+        update(k + 1, B);
+    } else if (k < k1) {
+        update(k, A);
     }
 }
 
@@ -615,22 +661,6 @@ __global__ __launch_bounds__(512) void str3d_elastic(StrParams3D P, PassExtras X
 // and the reads of the plane before.  The updates themselves are those of the per-plane
 // pass (shared helpers).
 
-// Loads through a buffer resource whose base is a wave-uniform plane pointer: the lane
-// offsets (bytes) are shared by all arrays of one tile geometry, so a loop over planes
-// keeps one offset register per geometry instead of a 64-bit address per array.  An
-// offset at or past `bytes` (the rest of the array from the base) reads 0.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float* a, int64_t s2, int p, int nplanes)
-{
-    const int64_t rest = 4 * s2 * (int64_t)(nplanes - p);
-    const int bytes = (int)(rest < 0x7fffffff ? rest : 0x7fffffff);
-    return __builtin_amdgcn_make_buffer_rsrc((void*)(a + s2 * p), (short)0, bytes, 0x00020000);
-}
-
-__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, uint32_t off)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
-}
-
 // Workgroup order of the staged passes: mode 0 the launch order, 1 each XCD one
 // contiguous range of the grid (xcd_remap), M > 1 windows of 8*M consecutive workgroups in
 // which each XCD takes a contiguous run of M, so that a tile's z and y neighbours mostly
@@ -684,15 +714,6 @@ struct PlaneStage {
         return 4u * (uint32_t)((zb + CE + t % H) + s1 * (y0 + t / H));
     }
 };
-
-// v, opaque to the optimiser: the addresses an update forms from its point (CPML strips,
-// rigid faces, injection) are then formed per plane of a marching loop instead of being
-// hoisted out of it, where each would hold a 64-bit register pair for the whole loop.
-__device__ __forceinline__ int opaque(int v)
-{
-    asm volatile("" : "+v"(v));
-    return v;
-}
 
 // ring of the last N samples of a column, oldest first
 template <int N>

@@ -176,6 +176,46 @@ def test_elastic_3d_variants_equal_the_oracle(built, env):
     assert r.returncode == 0 and "3-D elastic cases equal the oracle" in r.stdout, (r.returncode, r.stderr[-3000:])
 
 
+ACOUSTIC3 = r"""
+import sys
+sys.path[:0] = [{pkg!r}, {orc!r}, {tests!r}]
+from geophyinv import _lib
+_lib.load({lib!r})
+import numpy as np
+import oracle as O
+from helpers import experiment3d, medium3d, rel_l2
+ctx = _lib.Context(0)
+FIN = ["p", "vx", "vy", "vz"]
+cases = [experiment3d(medium3d(0, n=(22, 18, 25)), nt=50, order=o, sfields=("p", "vz"), rfields=("p", "vx", "vy", "vz"),
+                      ctx=ctx, seed=20 + o) for o in (2, 4, 8)]
+cases.append(experiment3d(medium3d(0, n=(21, 19, 23)), nt=50, order=2, sfields=("p",), rfields=("p", "vz"),
+                          pml_faces=["zmax", "ymin", "ymax", "xmin", "xmax"], stressfree_faces=["zmin"], ctx=ctx,
+                          seed=27))
+for k, pa in enumerate(cases):
+    orc = O.from_experiment(pa)
+    orc.run()
+    recs, fin = pa.run_shot(0, final_fields=FIN)
+    for q, f in enumerate(pa.rfields):
+        assert rel_l2(recs[f], orc.records(q)) == 0.0, (k, f, rel_l2(recs[f], orc.records(q)))
+    for f in FIN:
+        assert np.array_equal(fin[f], orc.field(f)), (k, f, rel_l2(fin[f], orc.field(f)))
+print("3-D acoustic cases equal the oracle", len(cases))
+"""
+
+
+@pytest.mark.parametrize("env", [{}, {"GPI_3D_MARCH": "1"}, {"GPI_3D_MARCH": "3", "GPI_3D_TY": "8"}],
+                         ids=["march-4", "march-1", "march-3x8"])
+def test_acoustic_3d_equals_the_oracle(built, env):
+    """The 3-D acoustic passes against the oracle at orders 2, 4, 8 and with a free surface,
+    marching over 4 planes per workgroup (the default: two planes per loop trip plus a
+    remainder), 1 plane (no trip) and 3 planes (a trip plus an odd remainder).  Records and
+    final wavefields equal bit for bit."""
+    code = ACOUSTIC3.format(pkg=os.path.join(ROOT, "geophyinv.jl_amd"), orc=os.path.join(ROOT, "oracle"),
+                            tests=os.path.join(ROOT, "tests"), lib=built)
+    r = run([sys.executable, "-c", code], timeout=900, env=env)
+    assert r.returncode == 0 and "3-D acoustic cases equal the oracle" in r.stdout, (r.returncode, r.stderr[-3000:])
+
+
 def test_elastic_orders_equal_the_oracle(built):
     """The elastic kernels against the oracle through the CPU build: 2-D at orders 2-8, free
     surface with velocity and stress sources beside it, rigid faces, padded columns whose
