@@ -4,7 +4,8 @@
 // and writes 3-6, the 2-D passes read 6-7 and write 2-3, a plain copy reads 1 and writes 1.
 // This program times out_j[i] = (j + 1) * sum_k in_k[i] over separately allocated arrays of
 // the same size, for several (K, M), with 4-, 8- and 16-byte accesses per lane (the
-// stencil kernels load 4 bytes per lane), and with a z-stencil's extra load: each lane also
+// stencil kernels load 4 bytes per lane; 4-byte lanes also with one element per thread,
+// as the stencil kernels have), and with a z-stencil's extra load: each lane also
 // reads the element just past its own (in_k[i + 1] for 4-byte lanes, the first float of
 // the next vector for wider ones), so that a pass's achieved bandwidth can be set beside
 // what a plain stream of the same number of arrays and the same access width reaches.
@@ -80,15 +81,16 @@ static float run(const Arrays& a, size_t nfloat, int reps, hipEvent_t e0, hipEve
 template <int K, int M, bool HALO>
 static int widths(const Arrays& a, size_t nfloat, int reps, hipEvent_t e0, hipEvent_t e1)
 {
+    const float t41 = run<K, M, 1, float, HALO>(a, nfloat, reps, e0, e1);
     const float t4 = run<K, M, 4, float, HALO>(a, nfloat, reps, e0, e1);
     const float t8 = run<K, M, 4, v2f, HALO>(a, nfloat, reps, e0, e1);
     const float t16 = run<K, M, 4, v4f, HALO>(a, nfloat, reps, e0, e1);
     if (hipGetLastError() != hipSuccess) return 1;
     const double bytes = (double)(K + M) * nfloat * sizeof(float);
-    printf("stream_ceiling read %2d write %d%s: %.3e bytes; GB/s with 4-byte lanes %.1f, 8-byte %.1f, "
-           "16-byte %.1f\n",
-           K, M, HALO ? " + z-neighbour" : "", bytes, bytes / (t4 * 1e-3) / 1e9, bytes / (t8 * 1e-3) / 1e9,
-           bytes / (t16 * 1e-3) / 1e9);
+    printf("stream_ceiling read %2d write %d%s: %.3e bytes; GB/s with 4-byte lanes, 1 element per thread %.1f, "
+           "4 per thread %.1f; 8-byte lanes %.1f, 16-byte %.1f (4 per thread)\n",
+           K, M, HALO ? " + z-neighbour" : "", bytes, bytes / (t41 * 1e-3) / 1e9, bytes / (t4 * 1e-3) / 1e9,
+           bytes / (t8 * 1e-3) / 1e9, bytes / (t16 * 1e-3) / 1e9);
     fflush(stdout);
     return 0;
 }
