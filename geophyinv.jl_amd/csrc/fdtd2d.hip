@@ -25,23 +25,26 @@ namespace gpi {
 
 namespace {
 
-// CPML memory update of cpml.jl:178-180 on a derivative that varies along x (column j of a
-// derivative grid with q.nd columns).  Min face first, then max face, as the reference
-// launches memorynpx! for xmin before xmax.
+// One CPML memory update (cpml.jl:178-180): psi = b psi + a d, d = d kI + psi, with every
+// load before the store (the compiler cannot tell that psi aliases none of the coefficient
+// arrays, so a coefficient read after the store would wait for it).
+__device__ __forceinline__ float cpml_step(float d, const Pml& q, float* m, int c)
+{
+    const float b = q.b[c], a = q.a[c], ki = q.kI[c], mo = *m;
+    const float mv = b * mo + a * d;
+    *m = mv;
+    return d * ki + mv;
+}
+
+// CPML of a derivative that varies along x (column j of a derivative grid with q.nd
+// columns).  Min face first, then max face, as the reference launches memorynpx! for xmin
+// before xmax.
 __device__ __forceinline__ float cpml_x(float d, const Pml& q, int i, int j, int npml, uint32_t mask)
 {
-    if ((mask & GPI_FACE(2, 0)) && j < npml) {
-        float* m = q.psi + i + q.ps2 * j;
-        float mv = q.b[j] * *m + q.a[j] * d;
-        *m = mv;
-        d = d * q.kI[j] + mv;
-    }
+    if ((mask & GPI_FACE(2, 0)) && j < npml) d = cpml_step(d, q, q.psi + i + q.ps2 * j, j);
     if ((mask & GPI_FACE(2, 1)) && j >= q.nd - npml) {
-        int c = j - (q.nd - npml) + npml;
-        float* m = q.psi + i + q.ps2 * c;
-        float mv = q.b[c] * *m + q.a[c] * d;
-        *m = mv;
-        d = d * q.kI[c] + mv;
+        const int c = j - (q.nd - npml) + npml;
+        d = cpml_step(d, q, q.psi + i + q.ps2 * c, c);
     }
     return d;
 }
@@ -49,18 +52,10 @@ __device__ __forceinline__ float cpml_x(float d, const Pml& q, int i, int j, int
 // Same along z (row i of a derivative grid with q.nd rows); psi rows are the 2*npml strip.
 __device__ __forceinline__ float cpml_z(float d, const Pml& q, int i, int j, int npml, uint32_t mask)
 {
-    if ((mask & GPI_FACE(0, 0)) && i < npml) {
-        float* m = q.psi + i + q.ps2 * j;
-        float mv = q.b[i] * *m + q.a[i] * d;
-        *m = mv;
-        d = d * q.kI[i] + mv;
-    }
+    if ((mask & GPI_FACE(0, 0)) && i < npml) d = cpml_step(d, q, q.psi + i + q.ps2 * j, i);
     if ((mask & GPI_FACE(0, 1)) && i >= q.nd - npml) {
-        int r = i - (q.nd - npml) + npml;
-        float* m = q.psi + r + q.ps2 * j;
-        float mv = q.b[r] * *m + q.a[r] * d;
-        *m = mv;
-        d = d * q.kI[r] + mv;
+        const int r = i - (q.nd - npml) + npml;
+        d = cpml_step(d, q, q.psi + r + q.ps2 * j, r);
     }
     return d;
 }

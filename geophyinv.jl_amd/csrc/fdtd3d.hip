@@ -19,41 +19,57 @@ namespace gpi {
 
 namespace {
 
+// EARLY: every load before the store (the compiler cannot tell that psi aliases none of
+// the coefficient arrays, so a coefficient read after the store waits for it): 0.917 /
+// 0.755 ms against 0.925 / 0.812 for the staged elastic passes on 360^3, stress pass of
+// 440^3 acoustic 0.458 against 0.478; the acoustic velocity pass measured 0.753 against
+// 0.735 that way and keeps kI after the store.
+template <bool EARLY = true>
 __device__ __forceinline__ float cpml_update(float d, const Pml& q, int idx, int c)
 {
     float* m = q.psi + idx;
-    const float mv = q.b[c] * *m + q.a[c] * d;
-    *m = mv;
-    return d * q.kI[c] + mv;
+    if constexpr (EARLY) {
+        const float b = q.b[c], a = q.a[c], ki = q.kI[c], mo = *m;
+        const float mv = b * mo + a * d;
+        *m = mv;
+        return d * ki + mv;
+    } else {
+        const float mv = q.b[c] * *m + q.a[c] * d;
+        *m = mv;
+        return d * q.kI[c] + mv;
+    }
 }
 
 // CPML of a derivative along x at derivative-grid point (i, j, k); q.nd = its x length
+template <bool EARLY = true>
 __device__ __forceinline__ float cpml3_x(float d, const Pml& q, int i, int j, int k, int npml, uint32_t mask)
 {
-    if ((mask & GPI_FACE(2, 0)) && k < npml) d = cpml_update(d, q, i + (int)q.ps1 * j + (int)q.ps2 * k, k);
+    if ((mask & GPI_FACE(2, 0)) && k < npml) d = cpml_update<EARLY>(d, q, i + (int)q.ps1 * j + (int)q.ps2 * k, k);
     if ((mask & GPI_FACE(2, 1)) && k >= q.nd - npml) {
         const int c = k - (q.nd - npml) + npml;
-        d = cpml_update(d, q, i + (int)q.ps1 * j + (int)q.ps2 * c, c);
+        d = cpml_update<EARLY>(d, q, i + (int)q.ps1 * j + (int)q.ps2 * c, c);
     }
     return d;
 }
 
+template <bool EARLY = true>
 __device__ __forceinline__ float cpml3_y(float d, const Pml& q, int i, int j, int k, int npml, uint32_t mask)
 {
-    if ((mask & GPI_FACE(1, 0)) && j < npml) d = cpml_update(d, q, i + (int)q.ps1 * j + (int)q.ps2 * k, j);
+    if ((mask & GPI_FACE(1, 0)) && j < npml) d = cpml_update<EARLY>(d, q, i + (int)q.ps1 * j + (int)q.ps2 * k, j);
     if ((mask & GPI_FACE(1, 1)) && j >= q.nd - npml) {
         const int r = j - (q.nd - npml) + npml;
-        d = cpml_update(d, q, i + (int)q.ps1 * r + (int)q.ps2 * k, r);
+        d = cpml_update<EARLY>(d, q, i + (int)q.ps1 * r + (int)q.ps2 * k, r);
     }
     return d;
 }
 
+template <bool EARLY = true>
 __device__ __forceinline__ float cpml3_z(float d, const Pml& q, int i, int j, int k, int npml, uint32_t mask)
 {
-    if ((mask & GPI_FACE(0, 0)) && i < npml) d = cpml_update(d, q, i + (int)q.ps1 * j + (int)q.ps2 * k, i);
+    if ((mask & GPI_FACE(0, 0)) && i < npml) d = cpml_update<EARLY>(d, q, i + (int)q.ps1 * j + (int)q.ps2 * k, i);
     if ((mask & GPI_FACE(0, 1)) && i >= q.nd - npml) {
         const int r = i - (q.nd - npml) + npml;
-        d = cpml_update(d, q, r + (int)q.ps1 * j + (int)q.ps2 * k, r);
+        d = cpml_update<EARLY>(d, q, r + (int)q.ps1 * j + (int)q.ps2 * k, r);
     }
     return d;
 }
@@ -215,7 +231,7 @@ __global__ __launch_bounds__(512) void vel3d_acoustic(VelParams3D P, PassExtras 
             for (int m = 0; m < O - 1; ++m) w[m] = w[m + 1];
             w[O - 1] = cur.wn;
             float d = stencil<O>(w) * P.dxI;
-            d = cpml3_x(d, P.qx[0], iu, ju, k, P.npml, P.pml_mask);
+            d = cpml3_x<false>(d, P.qx[0], iu, ju, k, P.npml, P.pml_mask);
             float nv = cur.vxo + cur.cxo * d;
             if ((P.rigid_mask & GPI_FACE(2, 0)) && k == 0) P.vx[(iu + H) + s1 * (ju + H)] = -nv;
             if ((P.rigid_mask & GPI_FACE(2, 1)) && k == P.nx - H - 1)
@@ -226,7 +242,7 @@ __global__ __launch_bounds__(512) void vel3d_acoustic(VelParams3D P, PassExtras 
         // vy on (nz-2h, ny-h, nx-2h): dpdy = D_yi(p)
         if (ycol && k < P.nx - 2 * H) {
             float d = stencil<O>(cur.py) * P.dyI;
-            d = cpml3_y(d, P.qy[0], iu, ju, k, P.npml, P.pml_mask);
+            d = cpml3_y<false>(d, P.qy[0], iu, ju, k, P.npml, P.pml_mask);
             float nv = cur.vyo + cur.cyo * d;
             if ((P.rigid_mask & GPI_FACE(1, 0)) && ju == 0) P.vy[(iu + H) + s2 * (k + H)] = -nv;
             if ((P.rigid_mask & GPI_FACE(1, 1)) && ju == P.ny - H - 1)
@@ -237,7 +253,7 @@ __global__ __launch_bounds__(512) void vel3d_acoustic(VelParams3D P, PassExtras 
         // vz on (nz-h, ny-2h, nx-2h): dpdz = D_zi(p)
         if (zcol && k < P.nx - 2 * H) {
             float d = stencil<O>(cur.pz) * P.dzI;
-            d = cpml3_z(d, P.qz[0], iu, ju, k, P.npml, P.pml_mask);
+            d = cpml3_z<false>(d, P.qz[0], iu, ju, k, P.npml, P.pml_mask);
             float nv = cur.vzo + cur.czo * d;
             if ((P.rigid_mask & GPI_FACE(0, 0)) && iu == 0) P.vz[s1 * (ju + H) + s2 * (k + H)] = -nv;
             if ((P.rigid_mask & GPI_FACE(0, 1)) && iu == P.nz - H - 1)
@@ -761,7 +777,9 @@ __global__ __launch_bounds__(64 * TY) void vel3d_elastic_staged(VelParams3D P, P
     }
     const int bx = b % P.gx, r0 = b / P.gx, by = r0 % P.gy, ch = r0 / P.gy;
     const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
-    const int i = bx * 64 + tx - H, j = by * TY + ty;   // rows shifted by h
+    // (j is uniform in a wave, whose 64 lanes are one row: read from the first lane, the
+    // y-strip CPML coefficients it indexes are scalar loads)
+    const int i = bx * 64 + tx - H, j = __builtin_amdgcn_readfirstlane(by * TY + ty);   // rows shifted by h
     const int k0 = ch * P.kc, k1 = min(k0 + P.kc, nxw);
     const int s1 = (int)P.s1, s2 = (int)P.s2;
     const int zb = bx * 64 - H, y0 = by * TY;
@@ -842,7 +860,7 @@ __global__ __launch_bounds__(64 * TY) void vel3d_elastic_staged(VelParams3D P, P
             v.e2 = de2[0];
             v.e3 = lds_z<O, W>(Lzz, ty + H, tx) * P.dzI;   // D_zi(tauzz) at (i, j+h, k+h)
             const int bt = bx + P.gx * (by + P.gy * k);
-            const int io = opaque(i), jo = opaque(j);
+            const int io = opaque(i), jo = __builtin_amdgcn_readfirstlane(opaque(j));
             vel3d_elastic_update<O>(P, X, io, jo, k, bt, (io + H) + s1 * (jo + H) + s2 * p, ux, uy, uz, v);
         }
         if (p < plast) {   // plane p+1 (loaded) into the other slot; loads for the next planes
@@ -872,7 +890,7 @@ __global__ __launch_bounds__(64 * TY) void str3d_elastic_staged(StrParams3D P, P
     }
     const int bx = b % P.gx, r0 = b / P.gx, by = r0 % P.gy, ch = r0 / P.gy;
     const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
-    const int i = bx * 64 + tx, j = by * TY + ty;
+    const int i = bx * 64 + tx, j = __builtin_amdgcn_readfirstlane(by * TY + ty);   // (uniform in a wave)
     const int k0 = ch * P.kc, k1 = min(k0 + P.kc, P.nx);
     const int s1 = (int)P.s1, s2 = (int)P.s2;
     const int zb = bx * 64, y0 = by * TY;
@@ -964,7 +982,7 @@ __global__ __launch_bounds__(64 * TY) void str3d_elastic_staged(StrParams3D P, P
             v.yza = lds_z<O, W>(Lvy, ty + H, tx) * P.dzI;    // D_zi(vy) at (i, j+h, k+h)
             v.yzc = lds_y<O, W>(Lvz, ty, tx + H) * P.dyI;    // D_yi(vz) at (i+h, j, k+h)
             const int bt = bx + P.gx * (by + P.gy * k);
-            const int io = opaque(i), jo = opaque(j);
+            const int io = opaque(i), jo = __builtin_amdgcn_readfirstlane(opaque(j));
             str3d_elastic_update<O>(P, X, io, jo, k, bt, io + s1 * jo + s2 * k, un, uxz, uxy, uyz, v);
         }
         if (p < plast) {   // plane p+1 (loaded) into the other slot; loads for the next planes
