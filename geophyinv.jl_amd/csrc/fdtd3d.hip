@@ -124,6 +124,18 @@ __device__ __forceinline__ int opaque(int v)
     return v;
 }
 
+// Workgroup order of the marching passes: mode 0 the launch order, 1 each XCD one
+// contiguous range of the grid (xcd_remap), M > 1 windows of 8*M consecutive workgroups in
+// which each XCD takes a contiguous run of M, so that a tile's z and y neighbours mostly
+// share its XCD's L2 while all XCDs stay in one region of the arrays.
+__device__ __forceinline__ int tile_order(int w, int n, int mode)
+{
+    if (mode == 0) return w;
+    if (mode == 1) return xcd_remap(w, n);
+    const int win = 8 * mode, base = w / win * win;
+    return base + xcd_remap(w - base, min(win, n - base));
+}
+
 struct Idx3 {
     int i, j, k;
 };
@@ -167,7 +179,7 @@ template <int O, bool NT>
 __global__ __launch_bounds__(512) void vel3d_acoustic(VelParams3D P, PassExtras X)
 {
     constexpr int H = O - 1;
-    const int b = P.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int b = tile_order((int)blockIdx.x, (int)gridDim.x, P.remap);
     const int nxw = P.nx - H;
     const int nch = (nxw + P.kc - 1) / P.kc;
     const int nbm = P.gx * P.gy * nch;
@@ -286,7 +298,7 @@ __global__ __launch_bounds__(512) void vel3d_acoustic(VelParams3D P, PassExtras 
 template <int O, bool NT>
 __global__ __launch_bounds__(512) void str3d_acoustic(StrParams3D P, PassExtras X)
 {
-    const int b = P.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int b = tile_order((int)blockIdx.x, (int)gridDim.x, P.remap);
     const int nch = (P.nx + P.kc - 1) / P.kc;
     const int nbm = P.gx * P.gy * nch;
     if (b >= nbm) {
@@ -677,18 +689,6 @@ __global__ __launch_bounds__(512) void str3d_elastic(StrParams3D P, PassExtras X
 // and the reads of the plane before.  The updates themselves are those of the per-plane
 // pass (shared helpers).
 
-// Workgroup order of the staged passes: mode 0 the launch order, 1 each XCD one
-// contiguous range of the grid (xcd_remap), M > 1 windows of 8*M consecutive workgroups in
-// which each XCD takes a contiguous run of M, so that a tile's z and y neighbours mostly
-// share its XCD's L2 while all XCDs stay in one region of the arrays.
-__device__ __forceinline__ int tile_order(int w, int n, int mode)
-{
-    if (mode == 0) return w;
-    if (mode == 1) return xcd_remap(w, n);
-    const int win = 8 * mode, base = w / win * win;
-    return base + xcd_remap(w - base, min(win, n - base));
-}
-
 // One x-plane of an array over a workgroup's tile as staged in workgroup memory: R = TY+h
 // rows of W = 64+h floats, column c holding z = zb + c.  The row loads of 64 lanes start
 // at column CM (memory z = zb + CM + lane, a 64-float boundary; lane offset offm of row
@@ -1057,6 +1057,20 @@ int elastic_planes(int ty)
 // order 4 on).  GPI_3D_EMARCH planes per workgroup (default 8); workgroup order
 // GPI_3D_XWIN = R > 0 (default 4) gives each XCD runs of R rows of tiles (tile_order),
 // 0 the order of GPI_XCD_REMAP3 (windows: 0.929 / 0.815 ms against 0.937 / 0.829).
+// Workgroup order of the acoustic passes: GPI_3D_AWIN = R > 0 (default 4) runs of R tile
+// rows per XCD (tile_order), 0 the order of GPI_XCD_REMAP3.  On 440^3 the windows cut the
+// measured traffic from 1.14x / 1.19x to 1.05x / 1.07x of the algorithmic bytes at the
+// same speed (0.738 / 0.455 ms; R = 2, 4 alike, 8 0.744 / 0.462).
+int acoustic_window()
+{
+    static int r = -1;
+    if (r < 0) {
+        const char* e = getenv("GPI_3D_AWIN");
+        r = e ? std::max(0, atoi(e)) : 4;
+    }
+    return r;
+}
+
 int elastic_window()
 {
     static int r = -1;
@@ -1116,7 +1130,7 @@ void launch_vel3(const Model& m, hipStream_t s, VelParams3D P, const PassExtras&
     }
     const int kz = m.g.physics == 0 ? 1 : elastic_planes(ty);
     P.kc = m.g.physics == 0 ? march_planes(P.nx - H) : kz;
-    P.remap = remap3();
+    P.remap = m.g.physics == 0 && acoustic_window() ? std::max(2, P.gx * acoustic_window()) : remap3();
     P.xfast = m.g.physics == 1 && elastic_xfast();
     dim3 grd(P.gx * P.gy * ((P.nx - H + P.kc - 1) / P.kc) + X.gather_blocks), blk(64, ty, kz);
     if (m.g.physics == 0 && nt_stores()) hipLaunchKernelGGL((vel3d_acoustic<O, true>), grd, blk, 0, s, P, X);
@@ -1141,7 +1155,7 @@ void launch_str3(const Model& m, hipStream_t s, StrParams3D P, const PassExtras&
     }
     const int kz = m.g.physics == 0 ? 1 : elastic_planes(ty);
     P.kc = m.g.physics == 0 ? march_planes(P.nx) : kz;
-    P.remap = remap3();
+    P.remap = m.g.physics == 0 && acoustic_window() ? std::max(2, P.gx * acoustic_window()) : remap3();
     P.xfast = m.g.physics == 1 && elastic_xfast();
     dim3 grd(P.gx * P.gy * ((P.nx + P.kc - 1) / P.kc) + X.gather_blocks), blk(64, ty, kz);
     if (m.g.physics == 0 && nt_stores()) hipLaunchKernelGGL((str3d_acoustic<O, true>), grd, blk, 0, s, P, X);
