@@ -203,13 +203,15 @@ print("3-D acoustic cases equal the oracle", len(cases))
 """
 
 
-@pytest.mark.parametrize("env", [{}, {"GPI_3D_MARCH": "1"}, {"GPI_3D_MARCH": "3", "GPI_3D_TY": "8"}],
+@pytest.mark.parametrize("env", [{}, {"GPI_3D_MARCH": "1", "GPI_3D_AWIN": "0"},
+                                 {"GPI_3D_MARCH": "3", "GPI_3D_TY": "8", "GPI_3D_AWIN": "1"}],
                          ids=["march-4", "march-1", "march-3x8"])
 def test_acoustic_3d_equals_the_oracle(built, env):
     """The 3-D acoustic passes against the oracle at orders 2, 4, 8 and with a free surface,
     marching over 4 planes per workgroup (the default: two planes per loop trip plus a
-    remainder), 1 plane (no trip) and 3 planes (a trip plus an odd remainder).  Records and
-    final wavefields equal bit for bit."""
+    remainder), 1 plane (no trip) and 3 planes (a trip plus an odd remainder), in the
+    default windowed workgroup order, the launch order and windows of one tile row.
+    Records and final wavefields equal bit for bit."""
     code = ACOUSTIC3.format(pkg=os.path.join(ROOT, "geophyinv.jl_amd"), orc=os.path.join(ROOT, "oracle"),
                             tests=os.path.join(ROOT, "tests"), lib=built)
     r = run([sys.executable, "-c", code], timeout=900, env=env)
