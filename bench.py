@@ -883,7 +883,7 @@ def main():
         workload = (f"3-D elastic FdtdElastic, {shape[0] - 40}^3 at 5 m, 20-cell CPML ({shape[0]}^3), three layers "
                     "vp 1800/2600/3500 m/s with a seeded dipping interface, explosive 25 Hz Ricker, 64x64 surface "
                     "receivers (vx, vy, vz), one shot per step")
-        kname, other_name = "str3d_elastic_staged<2, 4> (stress pass)", "velocity_pass"
+        kname, other_name = "str3d_elastic_staged<2, 4, true> (stress pass)", "velocity_pass"
         trf = measured_traffic("fdtd3d_elastic_360_traffic.md", "str3d_elastic_staged")
     elif three:
         metric = "Gpt-updates/s (3-D acoustic FDTD, FP32)"

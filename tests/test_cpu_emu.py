@@ -160,16 +160,18 @@ print("3-D elastic cases equal the oracle", len(cases))
                                  {"GPI_3D_ESTAGE": "2", "EMU_REVERSE": "1", "GPI_3D_XWIN": "0"},
                                  {"GPI_3D_ESTAGE": "2", "EMU_REVERSE": "1", "GPI_3D_TYE": "2", "GPI_XCD_REMAP3": "1",
                                   "GPI_3D_XWIN": "0"},
-                                 {"GPI_3D_ESTAGE": "0"}],
-                         ids=["staged-3x8", "staged-1x2", "staged-reverse", "staged-reverse-2", "per-plane"])
+                                 {"GPI_3D_ZPRE": "0"}, {"GPI_3D_ESTAGE": "0"}],
+                         ids=["staged-3x8", "staged-1x2", "staged-reverse", "staged-reverse-2", "staged-no-zpre",
+                              "per-plane"])
 def test_elastic_3d_variants_equal_the_oracle(built, env):
     """The 3-D elastic passes in their other forms against the oracle: the passes staged
     through workgroup memory at every order (by default only order 2 is staged) with other
     chunk lengths (3 planes: partial last chunk; 1 plane: every chunk only its warm-up plus
     one), tile heights (8 and 2 rows: the y halo rows loaded by one and by several waves),
-    workgroup orders (windows of 1 tile row, launch order, XCD ranges) and with the threads
+    workgroup orders (windows of 1 tile row, launch order, XCD ranges), with the threads
     of a workgroup run last first between barriers (a staging slot overwritten before every
-    thread has read it would show), and the per-plane passes at every order."""
+    thread has read it would show), without the z-strip CPML taken from the passes' own
+    loads, and the per-plane passes at every order."""
     code = ELASTIC3.format(pkg=os.path.join(ROOT, "geophyinv.jl_amd"), orc=os.path.join(ROOT, "oracle"),
                            tests=os.path.join(ROOT, "tests"), lib=built)
     r = run([sys.executable, "-c", code], timeout=900, env=env)
