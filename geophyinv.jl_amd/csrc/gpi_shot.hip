@@ -228,8 +228,9 @@ int plan_receivers(gpi_ctx* c, const Model& m, const State& st, Scratch& S, cons
         int& mx = vfield ? smax : vmax;
         mx = std::max(mx, P.npts);
     }
-    XV.gather_blocks = XV.g.nset * ((vmax + GATHER_PER_BLOCK - 1) / GATHER_PER_BLOCK);
-    XS.gather_blocks = XS.g.nset * ((smax + GATHER_PER_BLOCK - 1) / GATHER_PER_BLOCK);
+    const int vper = gather_per_block(XV.g.nnz), sper = gather_per_block(XS.g.nnz);
+    XV.gather_blocks = XV.g.nset * ((vmax + vper - 1) / vper);
+    XS.gather_blocks = XS.g.nset * ((smax + sper - 1) / sper);
     return GPI_OK;
 }
 

@@ -104,27 +104,50 @@ __device__ __forceinline__ float inject_velocity(const PassExtras& X, int kind, 
     return inject_velocity(X, X.it, kind, b, off, v, rho, dt);
 }
 
-// Receivers per gather workgroup, whatever the workgroup size: gather_blocks =
-// nset * ceil(max npts / GATHER_PER_BLOCK).
-constexpr int GATHER_PER_BLOCK = 256;
+// Receiver sampling by wavefront shuffles: the nnz taps of a receiver (its interpolation
+// offsets and weights, 4 in 2-D and 8 in 3-D) go to gather_group(nnz) consecutive lanes of
+// one wave, so that a wave reads the offsets, weights and field samples of its receivers
+// with one load instruction each (the table reads coalesced); the lane of tap 0 then sums
+// the products in tap order from its neighbours' registers, the same arithmetic as a loop
+// over the taps (src/fdtd/receiver.jl:3-14).  A gather workgroup samples
+// gather_per_block(nnz) receivers, one round for 256 threads: gather_blocks =
+// nset * ceil(max npts / gather_per_block(nnz)).  The host admits nnz <= 64 only.
+constexpr int GATHER_LANES = 256;
 
-// receiver sampling by workgroup gb of the extra range (one lane per receiver)
+__host__ __device__ constexpr int gather_group(int nnz)
+{
+    int g = 1;
+    while (g < nnz) g <<= 1;
+    return g;
+}
+
+__host__ __device__ constexpr int gather_per_block(int nnz) { return GATHER_LANES / gather_group(nnz); }
+
+// receiver sampling by workgroup gb of the extra range; every thread of the workgroup takes
+// part (the shuffles read the lanes of whole waves)
 __device__ __forceinline__ void gather_block(const PassExtras& X, int64_t it, int gb)
 {
     const int per = (X.gather_blocks + X.g.nset - 1) / X.g.nset;
     const int s = gb / per;
     if (s >= X.g.nset) return;
+    const int nnz = X.g.nnz, G = gather_group(nnz);
     const int nth = (int)(blockDim.x * blockDim.y);
-    const int lane = (int)(threadIdx.y * blockDim.x + threadIdx.x);
-    const int base = (gb - s * per) * GATHER_PER_BLOCK;
-    const int stop = min(base + GATHER_PER_BLOCK, X.g.npts[s]);
+    const int t = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    const int q = t & (G - 1);   // this lane's tap
+    const int base = (gb - s * per) * gather_per_block(nnz);
+    const int stop = min(base + gather_per_block(nnz), X.g.npts[s]);
     const float* F = X.g.field[s];
-    for (int r = base + lane; r < stop; r += nth) {
-        const int64_t* ix = X.g.idx[s] + (int64_t)r * X.g.nnz;
-        const float* ww = X.g.w[s] + (int64_t)r * X.g.nnz;
+    for (int r0 = base; r0 < stop; r0 += nth / G) {   // (bounds uniform over the workgroup)
+        const int r = r0 + t / G;
+        float w = 0.0f, f = 0.0f;
+        if (r < stop && q < nnz) {
+            const int64_t e = (int64_t)r * nnz + q;
+            w = X.g.w[s][e];
+            f = F[X.g.idx[s][e]];
+        }
         float tmp = 0.0f;
-        for (int k = 0; k < X.g.nnz; ++k) tmp += ww[k] * F[ix[k]];
-        X.g.rec[s][it * (int64_t)X.g.npts[s] + r] = tmp;
+        for (int m = 0; m < nnz; ++m) tmp += __shfl_down(w, m, G) * __shfl_down(f, m, G);
+        if (q == 0 && r < stop) X.g.rec[s][it * (int64_t)X.g.npts[s] + r] = tmp;
     }
 }
 

@@ -127,6 +127,22 @@ using std::min;
 // Between barriers the threads run in thread order, or last first with EMU_REVERSE=1.
 void __syncthreads();
 
+// Wave shuffle: the value of the lane d above this one within its segment of `width` lanes
+// (its own value past the segment's end), lanes being the workgroup's linear thread ids
+// (64 to a wave).  Exchanged through the workgroup between two barriers, so every thread of
+// the workgroup must call it the same number of times (as the kernels' uniform uses do).
+template <typename T>
+inline T __shfl_down(T v, unsigned int d, int width = 64)
+{
+    static thread_local T x[1024];
+    const int t = (int)(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
+    x[t] = v;
+    __syncthreads();
+    const T r = (t % width) + (int)d < width ? x[t + (int)d] : v;
+    __syncthreads();
+    return r;
+}
+
 // ---- runtime API --------------------------------------------------------------------
 hipError_t hipGetDeviceCount(int* n);
 hipError_t hipSetDevice(int d);
