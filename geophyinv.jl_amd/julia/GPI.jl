@@ -447,6 +447,33 @@ function mod_x_proc!(pac, pap, activepw, src_flags)
     return nothing
 end
 
+# ---- pullback of lossvalue (no reference counterpart) ----------------------------------
+# The adjoint half of gradient! (src/fdtd/func_grad.jl:21-45) for a reverse rule whose
+# primal is lossvalue (func_grad.jl:1-9): that :forward_save run left one store per
+# supersource in STORES, so only the :adjoint run is made here (mod_x_proc! consumes the
+# stores), followed by the same chain rule to the log-parameter vector g.
+function backprop!(g, m, loss, dobs, pa, mparams)
+    pac = pa.c
+    haskey(STORES, pac) && !isempty(STORES[pac]) ||
+        error("run lossvalue (the :forward_save mode) before its pullback")
+    mode_save = pac.attrib_mod.mode
+    GP.gradient!(pac.srcwav[2], loss, dobs, pac.data[1])   # adjoint sources
+    reverse!(pac.srcwav[2])
+    GP.update!(pa, pac.srcwav, [-1, 1], verbose = false)
+    pac.attrib_mod.mode = :adjoint
+    GP.update!(pa)
+    mchunks = GP.chunk(m, length(mparams))
+    gchunks = GP.chunk(g, length(mparams))
+    for (i, mname) in enumerate(mparams)
+        r = pac.ref_mod[mname]
+        gm = GP.view_inner(pac.gradients[mname], GP._fd_npextend, pac.pml_faces)
+        map!((gm1, x1) -> gm1 * exp(x1) * r, gm, gm, mchunks[i])
+        copyto!(gchunks[i], gm)
+    end
+    pac.attrib_mod.mode = mode_save
+    return g
+end
+
 # ---- the FWI outer loop (src/fwi/fwi.jl) -----------------------------------------------
 
 # The filter estimate of update!(paw, pac) (fwi.jl:42-57) in place of Conv.operator +

@@ -95,6 +95,21 @@ def test_binding_has_no_placeholder_bodies():
     assert not re.search(r"^\s*(#\s*)?\.\.\.", JL, re.M)
 
 
+def test_reverse_rule_calls_functions_of_the_binding():
+    """GPIChainRules.jl (the ChainRulesCore rule of lossvalue) calls only functions that
+    GPI.jl defines, and its pullback is the adjoint half of gradient! (func_grad.jl:21-45):
+    adjoint sources, time reversal, the :adjoint run, the exp(m) * ref_mod chain rule."""
+    rules = open(os.path.join(ROOT, "geophyinv.jl_amd", "julia", "GPIChainRules.jl")).read()
+    assert not re.search(r"^\s*(#\s*)?\.\.\.", rules, re.M)
+    defined = set(re.findall(r"^function (\w+!?)\(", JL, re.M))
+    used = set(re.findall(r"\bGPI\.(\w+!?)\(", rules))
+    assert used and used <= defined, used - defined
+    body = re.search(r"function backprop!\((.*?)\nend", JL, re.S).group(1)
+    for step in ("GP.gradient!(pac.srcwav[2]", "reverse!(pac.srcwav[2])", "[-1, 1]", ":adjoint",
+                 "exp(x1) * r"):
+        assert step in body, step
+
+
 def c_prototypes():
     """Entry point name -> number of parameters, from the header's prototypes."""
     out = {}
