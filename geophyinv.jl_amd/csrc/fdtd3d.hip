@@ -852,11 +852,9 @@ __global__ __launch_bounds__(64 * TY) void vel3d_elastic_staged(VelParams3D P, P
     const uint32_t offe = S::extra_offset(zb, y0, s1);
     const uint32_t offo = offm + 4u * (uint32_t)(s1 * H);   // (i+h, j+h)
     S st[6];
-    float xxn = 0.0f;   // tauxx at (i+h, j+h) of the plane in st
     auto stage = [&](int p) {
 #pragma unroll
         for (int a = 0; a < 6; ++a) st[a].load(plane_rsrc(src[a], s2, p, np), offm, rstep, offe);
-        xxn = bload(plane_rsrc(P.s[0], s2, p, np), offo);
     };
     // x rings (planes k .. k+h): tauxx (i+h, j+h), tauxy (i, j), tauxz (i, j), rho (i+h, j+h);
     // delay lines of D_ya(tauxy), D_za(tauxz), D_za(tauyz), D_ya(tauyz) at (i, j)
@@ -867,9 +865,13 @@ __global__ __launch_bounds__(64 * TY) void vel3d_elastic_staged(VelParams3D P, P
     // complete in order, so a use waits for every load issued before it.  v's loaded
     // samples and st are written only by their loads (a register that a pending load may
     // still write is not assigned otherwise, which would wait for every load in flight).
+    // (tauxx is loaded with the plane's own samples, one plane ahead like every operand: a
+    // register loaded two planes ahead would be copied at the loop's back edge after the
+    // next loads are issued, and that copy waits for all of them)
     VelElSamples v{};
-    float xx = 0.0f;   // tauxx at (i+h, j+h) of the staged plane
-    auto own = [&](int q) {   // velocities of the update of plane q - h (at plane q)
+    float xx = 0.0f;   // tauxx at (i+h, j+h) of plane q
+    auto own = [&](int q) {   // tauxx of plane q; velocities of the update of plane q - h
+        xx = bload(plane_rsrc(P.s[0], s2, q, np), offo);
         const int k = q - H;
         if (k < k0 || k >= k1) return;
         if (cx) v.vxo = bload(plane_rsrc(P.vx, s2, q, np), offo);
@@ -881,7 +883,6 @@ __global__ __launch_bounds__(64 * TY) void vel3d_elastic_staged(VelParams3D P, P
         stage(k0);
 #pragma unroll
         for (int a = 0; a < 6; ++a) st[a].store(L[0][a]);
-        xx = xxn;
         own(k0);
         if (k0 < plast) stage(k0 + 1);
     }
@@ -923,9 +924,8 @@ __global__ __launch_bounds__(64 * TY) void vel3d_elastic_staged(VelParams3D P, P
         if (p < plast) {   // plane p+1 (loaded) into the other slot; loads for the next planes
 #pragma unroll
             for (int a = 0; a < 6; ++a) st[a].store(L[slot ^ 1][a]);
-            xx = xxn;
             own(p + 1);
-            if (p + 1 < plast) stage(p + 2);
+            stage(min(p + 2, plast));   // (also on the last trip: the waits then count these loads on every path)
         }
     }
 }
@@ -1060,7 +1060,7 @@ __global__ __launch_bounds__(64 * TY) void str3d_elastic_staged(StrParams3D P, P
 #pragma unroll
             for (int a = 0; a < 4; ++a) st[a].store(L[slot ^ 1][a]);
             own(p + 1);
-            if (p + 1 < plast) stage(p + 2);
+            stage(min(p + 2, plast));   // (also on the last trip: the waits then count these loads on every path)
         }
     }
 }
