@@ -385,7 +385,7 @@ __global__ __launch_bounds__(64 * TY) void vel2d_kernel(VelParams2D P, PassExtra
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int nbm = P.gx * P.gy;
     if (b >= nbm) {
-        gather_block(X, X.it, b - nbm);
+        gather_block<4>(X, X.it, b - nbm);
         return;
     }
     vel_tile<O, PHYS, TY, RX>(P, X, X.it, X.ninj, b);
@@ -397,7 +397,7 @@ __global__ __launch_bounds__(64 * TY) void str2d_kernel(StrParams2D P, PassExtra
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int nbm = P.gx * P.gy;
     if (b >= nbm) {
-        gather_block(X, X.it, b - nbm);
+        gather_block<4>(X, X.it, b - nbm);
         return;
     }
     str_tile<O, PHYS, TY, RX>(P, X, X.it, X.ninj, b);
@@ -481,7 +481,7 @@ __device__ __forceinline__ void vel_batch_block(const BatchShot2D* __restrict__ 
     const int lb = b - first[s];
     const int nbm = S.vp.gx * S.vp.gy;
     if (lb >= nbm) {
-        if (it < S.nt) gather_block(S.xv, it, lb - nbm);
+        if (it < S.nt) gather_block<4>(S.xv, it, lb - nbm);
         return;
     }
     vel_tile<O, PHYS, TY, RX>(S.vp, S.xv, it, it < S.nt ? S.xv.ninj : 0, lb);
@@ -499,7 +499,7 @@ __device__ __forceinline__ void str_batch_block(const BatchShot2D* __restrict__ 
     const int lb = b - first[s];
     const int nbm = S.sp.gx * S.sp.gy;
     if (lb >= nbm) {
-        gather_block(S.xs, it, lb - nbm);
+        gather_block<4>(S.xs, it, lb - nbm);
         return;
     }
     str_tile<O, PHYS, TY, RX>(S.sp, S.xs, it, S.xs.ninj, lb);

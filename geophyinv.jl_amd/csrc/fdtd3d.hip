@@ -184,7 +184,7 @@ __global__ __launch_bounds__(512) void vel3d_acoustic(VelParams3D P, PassExtras 
     const int nch = (nxw + P.kc - 1) / P.kc;
     const int nbm = P.gx * P.gy * nch;
     if (b >= nbm) {
-        gather_block(X, b - nbm);
+        gather_block<8>(X, b - nbm);
         return;
     }
     const int bx = b % P.gx, r = b / P.gx, by = r % P.gy, ch = r / P.gy;
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(512) void str3d_acoustic(StrParams3D P, PassExtras 
     const int nch = (P.nx + P.kc - 1) / P.kc;
     const int nbm = P.gx * P.gy * nch;
     if (b >= nbm) {
-        gather_block(X, b - nbm);
+        gather_block<8>(X, b - nbm);
         return;
     }
     const int bx = b % P.gx, r = b / P.gx, by = r % P.gy, ch = r / P.gy;
@@ -495,7 +495,7 @@ __global__ __launch_bounds__(512) void vel3d_elastic(VelParams3D P, PassExtras X
     const int kz = (int)blockDim.z;
     const int nch = (nxw + kz - 1) / kz;
     if (b >= P.gx * P.gy * nch) {
-        if (threadIdx.z == 0) gather_block(X, b - P.gx * P.gy * nch);
+        if (threadIdx.z == 0) gather_block<8>(X, b - P.gx * P.gy * nch);
         return;
     }
     Idx3 t;
@@ -659,7 +659,7 @@ __global__ __launch_bounds__(512) void str3d_elastic(StrParams3D P, PassExtras X
     const int kz = (int)blockDim.z;
     const int nch = (P.nx + kz - 1) / kz;
     if (b >= P.gx * P.gy * nch) {
-        if (threadIdx.z == 0) gather_block(X, b - P.gx * P.gy * nch);
+        if (threadIdx.z == 0) gather_block<8>(X, b - P.gx * P.gy * nch);
         return;
     }
     Idx3 t;
@@ -829,7 +829,7 @@ __global__ __launch_bounds__(64 * TY) void vel3d_elastic_staged(VelParams3D P, P
     const int nch = (nxw + P.kc - 1) / P.kc;
     const int nbm = P.gx * P.gy * nch;
     if (b >= nbm) {
-        gather_block(X, b - nbm);
+        gather_block<8>(X, b - nbm);
         return;
     }
     const int bx = b % P.gx, r0 = b / P.gx, by = r0 % P.gy, ch = r0 / P.gy;
@@ -943,7 +943,7 @@ __global__ __launch_bounds__(64 * TY) void str3d_elastic_staged(StrParams3D P, P
     const int nch = (P.nx + P.kc - 1) / P.kc;
     const int nbm = P.gx * P.gy * nch;
     if (b >= nbm) {
-        gather_block(X, b - nbm);
+        gather_block<8>(X, b - nbm);
         return;
     }
     const int bx = b % P.gx, r0 = b / P.gx, by = r0 % P.gy, ch = r0 / P.gy;

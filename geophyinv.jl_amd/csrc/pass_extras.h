@@ -124,13 +124,15 @@ __host__ __device__ constexpr int gather_group(int nnz)
 __host__ __device__ constexpr int gather_per_block(int nnz) { return GATHER_LANES / gather_group(nnz); }
 
 // receiver sampling by workgroup gb of the extra range; every thread of the workgroup takes
-// part (the shuffles read the lanes of whole waves)
+// part (the shuffles read the lanes of whole waves).  NNZ: the taps per receiver when the
+// caller's dimension fixes them (the host admits 4 in 2-D and 8 in 3-D only), 0 to read them.
+template <int NNZ = 0>
 __device__ __forceinline__ void gather_block(const PassExtras& X, int64_t it, int gb)
 {
     const int per = (X.gather_blocks + X.g.nset - 1) / X.g.nset;
     const int s = gb / per;
     if (s >= X.g.nset) return;
-    const int nnz = X.g.nnz, G = gather_group(nnz);
+    const int nnz = NNZ ? NNZ : X.g.nnz, G = gather_group(nnz);
     const int nth = (int)(blockDim.x * blockDim.y);
     const int t = (int)(threadIdx.y * blockDim.x + threadIdx.x);
     const int q = t & (G - 1);   // this lane's tap
@@ -151,6 +153,7 @@ __device__ __forceinline__ void gather_block(const PassExtras& X, int64_t it, in
     }
 }
 
-__device__ __forceinline__ void gather_block(const PassExtras& X, int gb) { gather_block(X, X.it, gb); }
+template <int NNZ = 0>
+__device__ __forceinline__ void gather_block(const PassExtras& X, int gb) { gather_block<NNZ>(X, X.it, gb); }
 
 }  // namespace gpi
