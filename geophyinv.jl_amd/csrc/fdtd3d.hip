@@ -1125,7 +1125,8 @@ int elastic_planes(int ty)
 // (ms per launch, stress / velocity): order 2 staged 0.937 / 0.829 against per-plane
 // 1.041 / 0.825; order 4 1.228 / 1.162 against 1.167 / 0.952, order 8 2.216 / 2.414
 // against 1.679 / 1.343 (the staged passes' rings and delay lines need 107-193 VGPRs from
-// order 4 on).  GPI_3D_EMARCH planes per workgroup (default 8); workgroup order
+// order 4 on).  GPI_3D_EMARCH planes per workgroup (default 6: 0.849 / 0.761 ms against
+// 0.856 / 0.763 with 8, 5 and 7 in between, 12 and 16 slower); workgroup order
 // GPI_3D_XWIN = R > 0 (default 4) gives each XCD runs of R rows of tiles (tile_order),
 // 0 the order of GPI_XCD_REMAP3 (windows: 0.929 / 0.815 ms against 0.937 / 0.829).
 // Workgroup order of the acoustic passes: GPI_3D_AWIN = R > 0 (default 4) runs of R tile
@@ -1182,7 +1183,7 @@ int elastic_march(int n)
     static int kc = -1;
     if (kc < 0) {
         const char* e = getenv("GPI_3D_EMARCH");
-        kc = e ? atoi(e) : 8;
+        kc = e ? atoi(e) : 6;
         if (kc < 1) kc = 1;
     }
     return std::min(kc, std::max(n, 1));
